@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""
+Benchmark of the framecor correlator on one MI355X.
+
+Workload: the README shape of the reference (720 x 1280 uint8 frames, window 64, search +-48, skip 16:
+2450 windows): 64 synthetic frames of a smoothed-noise texture with a 30-degree fault, camera shake, a
+40 px slip ramp and two fixed text boxes (slipcam_amd.synthetic.fault_video), resident in HBM before
+the timed region. A step is one correlate() of the 64 frames (summed-area tables, box sums and the
+correlation kernel with its fused peak epilogue), device-resident in and out.
+
+Both forms are timed: int8 (exact int8-MFMA correlation; the headline value) and fft (the reference's
+FP32 FFT algorithm). The line also carries the per-stage GPU times from hipEvents on the stream of the
+calls, the roofline of the int8 correlation kernel, set_reference times, the PCIe-inclusive rate of the
+host API, and a CPU baseline: the oracle's restatement of the reference CPU algorithm (FP32 FFT,
+OpenMP over windows) on a bounded sample of the same frames.
+
+  python bench.py [--steps K] [--warmup W] [--frames 64] [--no-cpu-baseline]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'ms/frame + windows/s @720p 2450 win 64² ±48: int8-MFMA vs FP32-FFT'
+INT8_PEAK_TOPS = 5000.0        # MI355X dense int8 MFMA, TOP/s
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_method(torch, framecor, d, dev, method, steps, warmup):
+    h, w = d['frames'].shape[1:]
+    W, R, skip = d['window'], d['search'], d['skip']
+    down, across = framecor.window_grid(h, w, W, R, skip)
+    cor = framecor.Correlator(h, w, down, across, window=W, search=R, method=method)
+    t0 = time.perf_counter()
+    cor.set_reference(d['ref'])
+    set_ref_ms = (time.perf_counter() - t0) * 1e3
+    n = dev.shape[0]
+    for _ in range(warmup):
+        cor.correlate(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cor.correlate(dev)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    # the same steps again with per-stage hipEvents (the stream of the calls)
+    cor.set_timing(True)
+    stages = dict(summed_area=0.0, box_sums=0.0, correlate=0.0, launches=0)
+    for _ in range(steps):
+        cor.correlate(dev)
+        t = cor.timing()
+        for k in stages:
+            stages[k] += t[k]
+    cor.set_timing(False)
+    # the host API once (PCIe-inclusive: frames in, dict out)
+    t0 = time.perf_counter()
+    cor.correlate(d['frames'])
+    host_ms = (time.perf_counter() - t0) * 1e3
+    nwin = len(down) * len(across)
+    ms_step = elapsed / steps * 1e3
+    return dict(cor=cor, nwin=nwin, n=n, ms_per_step=ms_step, ms_per_frame=ms_step / n,
+                frames_per_s=n * steps / elapsed, windows_per_s=nwin * n * steps / elapsed,
+                set_reference_ms=set_ref_ms, host_api_ms_per_frame=host_ms / n,
+                stage_ms_per_frame={k: stages[k] / (steps * n) for k in ('summed_area', 'box_sums', 'correlate')},
+                kernel_ms_per_launch=stages['correlate'] / max(stages['launches'], 1),
+                frames_per_launch=n * steps / max(stages['launches'], 1))
+
+
+def cpu_baseline(d, seconds):
+    import oracle
+    from slipcam_amd import framecor
+    h, w = d['frames'].shape[1:]
+    W, R, skip = d['window'], d['search'], d['skip']
+    down, across = framecor.window_grid(h, w, W, R, skip)
+    o = oracle.OracleCorrelator(h, w, down, across, window=W, search=R)
+    o.set_reference(d['ref'])
+    done, t0 = 0, time.perf_counter()
+    while done < len(d['frames']) and (done == 0 or time.perf_counter() - t0 < seconds):
+        o.correlate_fft(d['frames'][done:done + 1])
+        done += 1
+    elapsed = time.perf_counter() - t0
+    nwin = len(down) * len(across)
+    return dict(value=round(nwin * done / elapsed, 1), unit='windows/s', cores=oracle.threads(), kind='port',
+                ms_per_frame=round(elapsed / done * 1e3, 2),
+                sample=f'{done} frames x {nwin} windows of the same workload, the oracle restatement of the '
+                       f'reference CPU algorithm (own FP32 FFT, no FFTW), OpenMP over windows')
+
+
+def main():
+    p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=10)
+    p.add_argument('--warmup', type=int, default=2)
+    p.add_argument('--frames', type=int, default=64)
+    p.add_argument('--cpu-seconds', type=float, default=15.0)
+    p.add_argument('--no-cpu-baseline', action='store_true')
+    args = p.parse_args()
+    if args.gpus != 1:
+        print('bench.py: this build measures one GPU only (the correlator is single-device; frames or '
+              'windows would shard across GPUs with no collective on the hot loop). Run with --gpus 1.',
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from slipcam_amd import framecor, synthetic
+
+    log('bench: generating the workload')
+    d = synthetic.fault_video(n=args.frames)
+    dev = torch.from_numpy(d['frames']).cuda()
+    torch.cuda.synchronize()
+    res = {}
+    for method in ('int8', 'fft'):
+        log(f'bench: timing {method}')
+        res[method] = time_method(torch, framecor, d, dev, method, args.steps, args.warmup)
+        log(f'bench: {method} {res[method]["ms_per_frame"]:.4f} ms/frame')
+    i8, ff = res['int8'], res['fft']
+    W, R, nwin = d['window'], d['search'], i8['nwin']
+    M = 2 * R + 1
+    ops_per_frame = 2.0 * W * W * M * M * nwin
+    ops_launch = ops_per_frame * i8['frames_per_launch']
+    achieved = ops_launch / (i8['kernel_ms_per_launch'] * 1e-3) / 1e12
+    L = W + 2 * R
+    spectra_bytes = nwin * L * (L // 2 + 1) * 8
+    fft_bytes_frame = spectra_bytes + d['frames'][0].nbytes + nwin * 7 * 4
+    fft_gbs = fft_bytes_frame * ff['frames_per_launch'] / (ff['kernel_ms_per_launch'] * 1e-3) / 1e9
+
+    line = {
+        'metric': METRIC,
+        'value': round(i8['windows_per_s'], 1),
+        'unit': 'windows/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(i8['ms_per_step'], 4),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'int8',
+        'data': 'synthetic',
+        'config': {'workload': f'fault_video {d["frames"].shape[1]}x{d["frames"].shape[2]} uint8, '
+                               f'{args.frames} frames per step, window {W}, search {R}, skip {d["skip"]} '
+                               f'({nwin} windows), frames resident in HBM',
+                   'frames_per_step': args.frames, 'windows': nwin},
+        'ms_per_frame': round(i8['ms_per_frame'], 5),
+        'frames_per_s': round(i8['frames_per_s'], 2),
+        'fft': {'ms_per_frame': round(ff['ms_per_frame'], 5), 'frames_per_s': round(ff['frames_per_s'], 2),
+                'windows_per_s': round(ff['windows_per_s'], 1), 'dtype': 'f32',
+                'stage_ms_per_frame': {k: round(v, 5) for k, v in ff['stage_ms_per_frame'].items()},
+                'kernel_ms_per_launch': round(ff['kernel_ms_per_launch'], 4),
+                'roofline': {'bound': 'hbm', 'achieved': round(fft_gbs, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                             'frac': round(fft_gbs / HBM_PEAK_GBS, 4)},
+                'set_reference_ms': round(ff['set_reference_ms'], 1),
+                'host_api_ms_per_frame': round(ff['host_api_ms_per_frame'], 4)},
+        'int8_over_fft': round(ff['ms_per_frame'] / i8['ms_per_frame'], 3),
+        'stage_ms_per_frame': {k: round(v, 5) for k, v in i8['stage_ms_per_frame'].items()},
+        'set_reference_ms': round(i8['set_reference_ms'], 1),
+        'host_api_ms_per_frame': round(i8['host_api_ms_per_frame'], 4),
+        'roofline': {'bound': 'mfma', 'achieved': round(achieved, 1), 'peak': INT8_PEAK_TOPS, 'unit': 'TOP/s',
+                     'frac': round(achieved / INT8_PEAK_TOPS, 4), 'traffic': None,
+                     'kernel': 'int8Correlate', 'kernel_ms_per_launch': round(i8['kernel_ms_per_launch'], 4),
+                     'ops_per_launch': ops_launch},
+    }
+    if not args.no_cpu_baseline:
+        log('bench: CPU baseline')
+        line['cpu_baseline'] = cpu_baseline(d, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,118 @@
+/*
+ * framecor_hip.h -- the C ABI of the MI355X framecor correlator (libframecor_hip.so)
+ *
+ * Dense normalized cross-correlation of 8-bit frames with a reference frame on a grid of square
+ * windows: the drop-in for the reference's native module slipcam.framecor._framecor, whose pybind11
+ * class the Python package slipcam_amd.framecor rebuilds on top of these entry points (ctypes).
+ * Plain C types only: pointers, sizes and status codes.
+ *
+ * Every entry point returns FRAMECOR_OK (0) or an error status; framecor_last_error() then holds the
+ * message (per thread). The statuses map to the exceptions of the reference module:
+ *   FRAMECOR_EINVAL   -> ValueError    (std::invalid_argument: cxx/common.cpp:17, cxx/bindings.cpp:44,50,61,79)
+ *   FRAMECOR_ERUNTIME -> RuntimeError  (std::runtime_error: cxx/bindings.cpp:59, cxx/cuda.cu:28-38)
+ *   FRAMECOR_ENOMEM   -> MemoryError   (std::bad_alloc: cxx/cpu.cpp:67)
+ *
+ * Outputs, for n frames and nwin = ndown * nacross windows, q = frame * nwin + window (windows of a
+ * grid row, then the next row), as cxx/framecor.h:37-43:
+ *   offset[2 q + 0..1]  (down, across) motion of the frame, NaN if not valid
+ *   peak[q]             normalized correlation at the peak (0 if not valid)
+ *   snr[q]              squared peak over the mean squared surface around it (0 if not valid)
+ *   cov[3 q + 0..2]     covariance of the offsets (down, across, cross); (99, 99, 0) if unknown
+ *
+ * A correlator is not re-entrant (it owns its tables and stream), as the reference's.
+ */
+#ifndef FRAMECOR_HIP_H
+#define FRAMECOR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FRAMECOR_OK = 0,
+    FRAMECOR_EINVAL = 1,
+    FRAMECOR_ERUNTIME = 2,
+    FRAMECOR_ENOMEM = 3
+};
+
+/* the correlation methods (a keyword the reference does not have; it only has the FFT) */
+enum {
+    FRAMECOR_METHOD_FFT = 0,    /* FP32 FFT, the reference algorithm */
+    FRAMECOR_METHOD_INT8 = 1    /* exact int8-MFMA correlation, FP64 peak fit */
+};
+
+typedef struct framecor_correlator framecor_correlator;
+
+/* the version string; replaces _framecor.__version__ (cxx/bindings.cpp:113, CMakeLists.txt:12) */
+const char *framecor_version(void);
+
+/* 1 if a GPU is visible; replaces has_cuda() (cxx/framecor.h:63, cxx/cuda.cu:383-387, cxx/bindings.cpp:115) */
+int framecor_has_gpu(void);
+
+/* the message of the last failing call on this thread ("" if none) */
+const char *framecor_last_error(void);
+
+/* checks the parameters of a correlator without touching a GPU: FRAMECOR_EINVAL with the message
+ * of the reference's checkParams (cxx/common.cpp:15-28) if they are not valid, or if the method does
+ * not support them */
+int framecor_check_params(int height, int width, const int *down, int ndown, const int *across, int nacross,
+                          int window, int search, int method, int batch, int stat_size);
+
+/* a correlator of height x width frames on the grid down[ndown] x across[nacross] of windows;
+ * replaces PyCorrelator::PyCorrelator (cxx/bindings.cpp:26-45) with checkParams
+ * (cxx/common.cpp:15-28) and makeCuda (cxx/cuda.cu:389-393). The parameters are checked before
+ * any GPU call (FRAMECOR_EINVAL with the reference's messages). batch (>= 1) and stat_size (>= 3)
+ * are the reference's keywords; method is FRAMECOR_METHOD_*. */
+int framecor_create(int height, int width, const int *down, int ndown, const int *across, int nacross,
+                    int window, int search, int method, int batch, int stat_size, int device,
+                    framecor_correlator **out);
+
+/* frees the correlator and its device memory (NULL is ignored) */
+void framecor_destroy(framecor_correlator *c);
+
+/* "hip"; replaces Correlator::backend (cxx/framecor.h:55, cxx/cuda.cu:299, cxx/bindings.cpp:99) */
+const char *framecor_backend(const framecor_correlator *c);
+
+/* the window grid (ndown, nacross); replaces the shape property (cxx/bindings.cpp:106, 139) */
+int framecor_shape(const framecor_correlator *c, int *ndown, int *nacross);
+
+/* sets the reference frame (height x width uint8, host memory) and caches what its windows need;
+ * may be called again to replace it. Replaces PyCorrelator::setReference (cxx/bindings.cpp:47-55)
+ * and Cuda::setReference (cxx/cuda.cu:301-315). Synchronous. */
+int framecor_set_reference(framecor_correlator *c, const uint8_t *frame);
+
+/* correlates n frames (n x height x width uint8, host memory) with the reference frame; gross:
+ * n x nwin x 2 int32 (down, across) integer offsets of the search windows, or NULL; outputs in host
+ * memory. Replaces PyCorrelator::correlate's call of Correlator::correlate (cxx/bindings.cpp:86-90,
+ * cxx/framecor.h:52-53, cxx/cuda.cu:317-360). Synchronous. FRAMECOR_ERUNTIME before a reference. */
+int framecor_correlate(framecor_correlator *c, const uint8_t *frames, int n, const int32_t *gross,
+                       float *offset, float *peak, float *snr, float *cov);
+
+/* the same with every array in device memory, enqueued on stream (a hipStream_t; NULL: the
+ * correlator's own stream) and not synchronized: the outputs are complete when the stream is. The
+ * reference has no device-resident entry point; this is the one its CUDA backend's batch loop
+ * (cxx/cuda.cu:329-359) runs between its copies. */
+int framecor_correlate_device(framecor_correlator *c, const uint8_t *frames, int n, const int32_t *gross,
+                              float *offset, float *peak, float *snr, float *cov, void *stream);
+
+/* per-stage GPU timing (no counterpart in the reference; for benchmarks): with enable 1, the
+ * correlate calls record hipEvents on their stream around each stage; framecor_get_timing waits for
+ * the last call's events and writes the milliseconds of its stages, summed over its groups of frames:
+ * ms[0] summed-area tables, ms[1] box sums, ms[2] the correlation kernel; and *launches, the number of
+ * correlation-kernel launches */
+int framecor_set_timing(framecor_correlator *c, int enable);
+int framecor_get_timing(framecor_correlator *c, float *ms, int *launches);
+
+/* the (h + 1) x (w + 1) int64 summed-area tables of n frames (device memory) and of their squares,
+ * row and column 0 zero; replaces satRows/satColumns (cxx/cuda.cu:72-110, cxx/cpu.cpp:24-42).
+ * Enqueued on stream (NULL: the null stream), not synchronized. */
+int framecor_summed_area_device(const uint8_t *frames, int n, int height, int width, int64_t *sat,
+                                int64_t *sat2, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FRAMECOR_HIP_H */

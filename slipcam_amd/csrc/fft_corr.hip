@@ -1,0 +1,277 @@
+// fft_corr.hip -- the FP32 FFT form: one block per (window, frame), the half spectrum in LDS
+//
+// Per search window (cxx/cpu.cpp:117-167 of the reference, as one fused kernel): the L x L chip minus
+// its mean (from the summed-area table) is loaded into LDS as packed row pairs; forward row passes,
+// the split into half spectra, forward column passes; the product with the conjugate of the cached
+// spectrum of the reference window (read from HBM once per window and frame, coalesced); inverse
+// column passes, the combine, inverse row passes over the M surface rows only; then, in LDS, the
+// normalization of the M x M surface with the box sums (cxx/surface.h:30-35: c / L^2 over
+// norm * sqrt(v / n)), the first maximum, and the reference's FP32 quadratic fit, FP64 SNR and
+// covariance. HBM traffic per window and frame: the chip, the reference spectrum, the box sums of the
+// surface and the outputs; nothing else leaves LDS.
+
+#include "common.h"
+#include "epilogue.h"
+#include "fft_phases.h"
+#include "kernels.h"
+
+namespace framecor {
+
+namespace {
+
+constexpr int kFftThreads = 512;
+constexpr int kIpt = 4;   // items per thread and round of the split and combine steps
+
+struct BlockSync {
+    __device__ void operator()() const { __syncthreads(); }
+};
+
+struct FftShared {
+    float2 *Z;      // np x rs
+    float2 *tw;     // L
+    int *pos;       // L
+    float *redv;    // waves
+    int *redi;      // waves
+    double *redd;   // waves
+};
+
+__device__ inline FftShared carve(unsigned char *lds, const FftArgs &a)
+{
+    FftShared s;
+    s.Z = reinterpret_cast<float2 *>(lds);
+    s.tw = s.Z + size_t(a.np) * a.rs;
+    s.redd = reinterpret_cast<double *>(s.tw + a.g.L);
+    s.pos = reinterpret_cast<int *>(s.redd + kFftThreads / kWave);
+    s.redv = reinterpret_cast<float *>(s.pos + a.g.L);
+    s.redi = reinterpret_cast<int *>(s.redv + kFftThreads / kWave);
+    return s;
+}
+
+__device__ inline void loadTables(const FftShared &s, const FftArgs &a, int tid)
+{
+    const int L = a.g.L;
+    for (int k = tid; k < L; k += kFftThreads) {
+        s.tw[k] = a.twiddle[k];
+        s.pos[k] = digitReversed(k, a.plan.fac, a.plan.nf, L);
+    }
+}
+
+// forward row passes, split, forward column passes of the packed data in s.Z
+__device__ inline void forward2d(const FftShared &s, const FftArgs &a, int tid)
+{
+    const int L = a.g.L;
+    BlockSync sync;
+    transformLines<false>(s.Z, s.tw, L, a.plan.fac, a.plan.nf, a.np, RowAddr{a.rs}, tid, kFftThreads, sync);
+    splitPairs<kIpt>(s.Z, s.pos, L, a.H, a.rs, a.np, tid, kFftThreads, sync);
+    transformLines<false>(s.Z, s.tw, L, a.plan.fac, a.plan.nf, a.H, ColAddr{a.rs, a.H}, tid, kFftThreads, sync);
+}
+
+// the spectra of the reference windows: one block per window
+__global__ void __launch_bounds__(kFftThreads) fftReference(FftArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const FftShared s = carve(lds, a);
+    const int tid = threadIdx.x, k = blockIdx.x;
+    const int W = a.g.W;
+    loadTables(s, a, tid);
+    // zero-padded template, packed: row y -> half (y & 1) of pair y >> 1
+    float *zf = reinterpret_cast<float *>(s.Z);
+    for (int i = tid; i < a.np * a.rs * 2; i += kFftThreads) zf[i] = 0.0f;
+    __syncthreads();
+    const float *T = a.templ + size_t(k) * W * W;
+    for (int idx = tid; idx < W * W; idx += kFftThreads) {
+        const int y = idx / W, x = idx - y * W;
+        zf[2 * ((y >> 1) * a.rs + x) + (y & 1)] = T[idx];
+    }
+    __syncthreads();
+    forward2d(s, a, tid);
+    float2 *out = a.spectraOut + size_t(k) * a.np * a.rs;
+    for (int i = tid; i < a.np * a.rs; i += kFftThreads) out[i] = s.Z[i];
+}
+
+__global__ void __launch_bounds__(kFftThreads) fftCorrelate(FftArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const FftShared s = carve(lds, a);
+    const Geometry &g = a.g;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
+    const int logical = xcdRemap(blockIdx.x, gridDim.x);
+    const int k = logical / a.nfb, fblock = logical - k * a.nfb;
+    const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n);
+    const int L = g.L, M = g.M, W = g.W;
+    BlockSync sync;
+
+    loadTables(s, a, tid);
+    const float normk = a.wd.norm[k];
+    const int y0 = a.wd.down[k / g.nx], x0 = a.wd.across[k % g.nx];
+    const float2 *ref = a.spectra + size_t(k) * a.np * a.rs;
+    float *zf = reinterpret_cast<float *>(s.Z);
+
+    for (int f = fBegin; f < fEnd; f++) {
+        const int64_t q = int64_t(f) * g.nwin + k;
+        int gy = 0, gx = 0;
+        if (a.fg.gross) {
+            gy = a.fg.gross[2 * q];
+            gx = a.fg.gross[2 * q + 1];
+        }
+        int sy, sx;
+        if (!(normk > 0) || !searchWindow(g, y0, x0, gy, gx, sy, sx)) {
+            if (tid == 0) invalidateOut(q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
+            continue;
+        }
+        // the chip minus its mean (the mean from the summed-area table, as cxx/cpu.cpp:130-137)
+        const int64_t *sat = a.fg.sat + f * g.satSize;
+        const int64_t st = g.w + 1;
+        const int64_t boxL = sat[(sy + L) * st + sx + L] - sat[sy * st + sx + L] - sat[(sy + L) * st + sx]
+            + sat[sy * st + sx];
+        const float mean = float(boxL) / (float(L) * float(L));
+        __syncthreads();   // the previous frame's epilogue read the surface
+        const uint8_t *src = a.fg.frames + size_t(f) * g.pixels + size_t(sy) * g.w + sx;
+        for (int idx = tid; idx < L * L; idx += kFftThreads) {
+            const int y = idx / L, x = idx - y * L;
+            zf[2 * ((y >> 1) * a.rs + x) + (y & 1)] = float(src[size_t(y) * g.w + x]) - mean;
+        }
+        if (L & 1)   // the imaginary half of the last pair: zero
+            for (int x = tid; x < L; x += kFftThreads) zf[2 * ((a.np - 1) * a.rs + x) + 1] = 0.0f;
+        __syncthreads();
+
+        forward2d(s, a, tid);
+        // times the conjugate spectrum of the reference window
+        for (int i = tid; i < a.np * a.rs; i += kFftThreads) s.Z[i] = cmulc(s.Z[i], ref[i]);
+        __syncthreads();
+        transformLines<true>(s.Z, s.tw, L, a.plan.fac, a.plan.nf, a.H, ColAddr{a.rs, a.H}, tid, kFftThreads, sync);
+        const int npOut = (M + 1) / 2;
+        combinePairs<kIpt>(s.Z, s.pos, L, a.H, a.rs, npOut, tid, kFftThreads, sync);
+        transformLines<true>(s.Z, s.tw, L, a.plan.fac, a.plan.nf, npOut, RowAddr{a.rs}, tid, kFftThreads, sync);
+
+        // normalize in place and find the first maximum
+        const float scale = 1.0f / (float(L) * float(L));
+        const int n = W * W;
+        const int32_t *box = a.fg.box + f * g.boxSize + int64_t(sy) * g.bw + sx;
+        const int32_t *box2 = a.fg.box2 + f * g.boxSize + int64_t(sy) * g.bw + sx;
+        float best = -2.0f;
+        int ib = 0x7fffffff;
+        for (int idx = tid; idx < M * M; idx += kFftThreads) {
+            const int u = idx / M, v = idx - u * M;
+            float &c = zf[2 * ((u >> 1) * a.rs + v) + (u & 1)];
+            const int64_t o = int64_t(u) * g.bw + v;
+            const int64_t bs = box[o], bs2 = box2[o];
+            const int64_t var = int64_t(n) * bs2 - bs * bs;
+            float val = 0.0f;
+            {
+#pragma clang fp contract(off)
+                const float cc = c * scale;
+                if (var > 0 && normk > 0) val = cc / (normk * sqrtf(float(var) / float(n)));
+            }
+            c = val;
+            bestOf(best, ib, val, idx);
+        }
+        waveBest(best, ib);
+        if (lane == 0) {
+            s.redv[wave] = best;
+            s.redi[wave] = ib;
+        }
+        __syncthreads();
+        best = -2.0f;
+        ib = 0x7fffffff;
+        for (int t = 0; t < kFftThreads / kWave; t++) bestOf(best, ib, s.redv[t], s.redi[t]);
+        if (!(best > 0)) {   // uniform
+            if (tid == 0) invalidateOut(q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
+            continue;
+        }
+        const int ub = ib / M, vb = ib - ub * M;
+        auto at = [&](int u, int v) { return zf[2 * ((u >> 1) * a.rs + v) + (u & 1)]; };
+        // the stat-box sum of squares (FP64)
+        const int nn = g.stat < M ? g.stat : M;
+        const int ulo = max(ub - nn / 2, 0), uhi = min(ub - nn / 2 + nn, M);
+        const int vlo = max(vb - nn / 2, 0), vhi = min(vb - nn / 2 + nn, M);
+        const int bwid = vhi - vlo, items = (uhi - ulo) * bwid;
+        double sum = 0.0;
+        for (int it = tid; it < items; it += kFftThreads) {
+            const int u = ulo + it / bwid, v = vlo + it % bwid;
+            const double c = at(u, v);
+            sum += c * c;
+        }
+        sum = waveSumD(sum);
+        if (lane == 0) s.redd[wave] = sum;
+        __syncthreads();
+        if (tid == 0) {
+            double total = 0.0;
+            for (int t = 0; t < kFftThreads / kWave; t++) total += s.redd[t];
+            const bool inside = ub > 0 && ub < M - 1 && vb > 0 && vb < M - 1;
+            float fs[3][3];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) fs[i][j] = inside ? at(ub + i - 1, vb + j - 1) : 0.0f;
+            fs[1][1] = at(ub, vb);
+            writePeakF32(fs, inside, ub, vb, g.R, gy, gx, total, items, double(n), q, a.fg.offset, a.fg.peak,
+                         a.fg.snr, a.fg.cov);
+        }
+    }
+}
+
+void configure(const void *kern)
+{
+    (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipGetLastError();
+}
+
+} // namespace
+
+bool fftPlanFor(int L, FftPlan &plan)
+{
+    plan.nf = 0;
+    int m = L;
+    // radix 8 first (fewest passes), then 4, 2, 5, 3, then any odd factor up to kMaxRadix
+    static const int first[5] = {8, 4, 2, 5, 3};
+    for (int r : first)
+        while (m % r == 0) {
+            if (plan.nf == kMaxFactors) return false;
+            plan.fac[plan.nf++] = r;
+            m /= r;
+        }
+    for (int r = 7; m > 1 && r <= kMaxRadix; r += 2)
+        while (m % r == 0) {
+            if (plan.nf == kMaxFactors) return false;
+            plan.fac[plan.nf++] = r;
+            m /= r;
+        }
+    return m == 1 && fftShmem(L) <= 160 * 1024;
+}
+
+size_t fftShmem(int L)
+{
+    const int np = (L + 1) / 2, rs = 2 * (L / 2 + 1);
+    return sizeof(float2) * size_t(np) * rs + sizeof(float2) * L + sizeof(double) * (kFftThreads / kWave)
+        + sizeof(int) * L + (sizeof(float) + sizeof(int)) * (kFftThreads / kWave);
+}
+
+void launchFftReference(const FftArgs &a, hipStream_t stream)
+{
+    static bool configured = false;
+    if (!configured) {
+        configure(reinterpret_cast<const void *>(fftReference));
+        configured = true;
+    }
+    hipLaunchKernelGGL(fftReference, dim3(a.g.nwin), dim3(kFftThreads), fftShmem(a.g.L), stream, a);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+void launchFftCorrelate(FftArgs a, hipStream_t stream)
+{
+    if (a.fg.n <= 0) return;
+    static bool configured = false;
+    if (!configured) {
+        configure(reinterpret_cast<const void *>(fftCorrelate));
+        configured = true;
+    }
+    // one frame per block: consecutive logical blocks are the frames of one window, so an XCD's
+    // blocks share that window's spectrum in its L2
+    a.fpb = 1;
+    a.nfb = a.fg.n;
+    const int64_t blocks = int64_t(a.g.nwin) * a.nfb;
+    if (blocks > 0x7fffffff) throw std::invalid_argument("framecor: too many windows x frames in one group");
+    hipLaunchKernelGGL(fftCorrelate, dim3(unsigned(blocks)), dim3(kFftThreads), fftShmem(a.g.L), stream, a);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+} // namespace framecor

@@ -1,0 +1,72 @@
+// kernels.h -- host-side launchers of the framecor kernels
+#pragma once
+
+#include "common.h"
+
+namespace framecor {
+
+// ---- summed-area tables and box sums (sat.hip)
+
+// the (h + 1) x (w + 1) int64 tables of n frames (n x h x w uint8), pixels and squares
+void launchSummedArea(const uint8_t *frames, int n, int h, int w, int64_t *sat, int64_t *sat2, hipStream_t stream);
+// the W x W box sums of n frames from their tables, n x (h - W + 1) x (w - W + 1) int32 each
+void launchBoxSums(const int64_t *sat, const int64_t *sat2, int n, int h, int w, int W, int32_t *box,
+                   int32_t *box2, hipStream_t stream);
+
+// ---- the int8-MFMA form (int8_corr.hip)
+
+struct Int8Args {
+    Geometry g;
+    WindowData wd;
+    FrameGroup fg;
+    const uint8_t *tmpl;   // nwin x W x W, the reference windows
+    int nut;               // 16-row bands of the surface: the wavefronts of a block
+    int ni4;               // template row groups of 4
+    int tstride;           // LDS bytes per template row
+    int cstride;           // LDS bytes per chip row
+    int crows;             // LDS chip rows
+    int fpb, nfb;          // frames per block, frame blocks
+};
+
+struct Int8Layout {
+    int njb, nvt, nut, ni4, tstride, cstride, crows;
+    size_t shmem;
+    bool supported;
+};
+
+Int8Layout int8Layout(const Geometry &g);
+void launchInt8(Int8Args a, const Int8Layout &l, hipStream_t stream);
+
+// ---- the FP32 FFT form (fft_corr.hip)
+
+constexpr int kMaxFactors = 12;
+
+struct FftPlan {
+    int nf;                  // number of radix passes
+    int fac[kMaxFactors];    // radices, in the order of the forward (decimation in frequency) passes
+};
+
+struct FftArgs {
+    Geometry g;
+    WindowData wd;
+    FrameGroup fg;
+    const float2 *spectra;   // nwin x np x rs: the spectra of the reference windows (kernel layout)
+    const float2 *twiddle;   // L: exp(-2 pi i k / L)
+    const float *templ;      // nwin x W x W zero-mean reference windows (set_reference only)
+    float2 *spectraOut;      // set_reference only
+    FftPlan plan;
+    int np;                  // row pairs: (L + 1) / 2
+    int H;                   // L / 2 + 1, the half-spectrum width
+    int rs;                  // complex slots per row pair: 2 H
+    int fpb, nfb;            // frames per block, frame blocks
+};
+
+// whether the FFT form supports L (factors <= 32 and the half spectrum in LDS), and its plan
+bool fftPlanFor(int L, FftPlan &plan);
+size_t fftShmem(int L);
+// the spectra of the reference windows (args: g, wd, twiddle, templ, spectraOut, plan, np, H, rs)
+void launchFftReference(const FftArgs &a, hipStream_t stream);
+// correlate a group of frames
+void launchFftCorrelate(FftArgs a, hipStream_t stream);
+
+} // namespace framecor

@@ -1,0 +1,275 @@
+"""
+framecor: dense normalized cross-correlation of 8-bit video frames with a reference frame, on an
+MI355X (hand-written HIP kernels for gfx950, libframecor_hip.so)
+
+The API of the reference's slipcam.framecor (python/slipcam/framecor/__init__.py, cxx/bindings.cpp):
+
+  grid = framecor.window_grid(720, 1280, window=64, search=48, skip=16)
+  cor = framecor.Correlator(720, 1280, *grid, window=64, search=48)
+  cor.set_reference(frames[ref])
+  result = cor.correlate(frames[first:last])     # offsets, peak, snr, cov
+
+plus the keyword method='fft' (the reference's FP32 FFT algorithm, the default) or 'int8' (exact
+integer correlation on the int8 matrix cores, FP64 peak fit), and device-resident inputs: correlate()
+also takes a torch uint8 tensor on the GPU and then returns torch tensors without a host copy.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ['Correlator', 'has_gpu', 'window_grid', '__version__']
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB_PATH = os.path.join(_PKG, 'libframecor_hip.so')
+
+_OK, _EINVAL, _ERUNTIME, _ENOMEM = 0, 1, 2, 3
+_METHODS = {'fft': 0, 'int8': 1}
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        raise ImportError(f'framecor: {_LIB_PATH} is not built (python -m slipcam_amd.build)')
+    lib = ctypes.CDLL(_LIB_PATH)
+    P, I = ctypes.c_void_p, ctypes.c_int
+    lib.framecor_version.restype = ctypes.c_char_p
+    lib.framecor_has_gpu.restype = I
+    lib.framecor_last_error.restype = ctypes.c_char_p
+    lib.framecor_check_params.restype = I
+    lib.framecor_check_params.argtypes = [I, I, P, I, P, I, I, I, I, I, I]
+    lib.framecor_create.restype = I
+    lib.framecor_create.argtypes = [I, I, P, I, P, I, I, I, I, I, I, I, ctypes.POINTER(P)]
+    lib.framecor_destroy.restype = None
+    lib.framecor_destroy.argtypes = [P]
+    lib.framecor_backend.restype = ctypes.c_char_p
+    lib.framecor_backend.argtypes = [P]
+    lib.framecor_set_reference.restype = I
+    lib.framecor_set_reference.argtypes = [P, P]
+    lib.framecor_correlate.restype = I
+    lib.framecor_correlate.argtypes = [P, P, I, P, P, P, P, P]
+    lib.framecor_correlate_device.restype = I
+    lib.framecor_correlate_device.argtypes = [P, P, I, P, P, P, P, P, P]
+    lib.framecor_set_timing.restype = I
+    lib.framecor_set_timing.argtypes = [P, I]
+    lib.framecor_get_timing.restype = I
+    lib.framecor_get_timing.argtypes = [P, P, P]
+    lib.framecor_summed_area_device.restype = I
+    lib.framecor_summed_area_device.argtypes = [P, I, I, I, P, P, P]
+    return lib
+
+
+_lib = _load()
+__version__ = _lib.framecor_version().decode()
+
+
+def _raise(status):
+    msg = _lib.framecor_last_error().decode()
+    if status == _EINVAL:
+        raise ValueError(msg)
+    if status == _ENOMEM:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+def _check(status):
+    if status != _OK:
+        _raise(status)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def has_gpu():
+    """Whether a GPU is visible to the HIP runtime (the reference's has_cuda)"""
+    return bool(_lib.framecor_has_gpu())
+
+
+def window_grid(height, width, window, search, skip, margin=0):
+    """
+    The first pixels (down, across) of the grid of windows whose search windows are in the frame:
+    from margin + search, (size - 2 (margin + search) - window) // skip windows in each direction;
+    a ValueError if no window fits (python/slipcam/framecor/__init__.py:18-32 of the reference)
+    """
+    axes = []
+    for direction, size in (('down', height), ('across', width)):
+        first = margin + search
+        count = (size - margin - search - first - window) // skip
+        if count <= 0:
+            raise ValueError(f'window_grid: no window fits {direction}: frame size {size}, window {window}, '
+                             f'search range {search}, margin {margin}')
+        axes.append(first + skip * np.arange(count))
+    return tuple(axes)
+
+
+def _is_device_tensor(x):
+    t = type(x)
+    return t.__module__.startswith('torch') and t.__name__ == 'Tensor' and x.is_cuda
+
+
+class Correlator:
+    """
+    The correlator of 8-bit frames with a reference frame, on a grid of square windows
+
+    height, width: the size of the frames
+    down, across: the first pixels of the rows and of the columns of windows
+    window: the size of the windows; search: the half search range, pixels
+    backend: 'hip' or 'auto' (the GPU); the reference's 'cuda' and 'cpu' are not backends here
+    method: 'fft' (the reference's FP32 FFT algorithm) or 'int8' (exact int8 matrix-core correlation)
+    batch: at least 1 (kept for the reference's API); stat_size: the SNR window
+    device: the GPU; threads: kept for the reference's API (the CPU threads of its CPU backend)
+    """
+
+    def __init__(self, height, width, down, across, *, window=64, search=48, backend='auto', batch=4096,
+                 stat_size=21, device=0, threads=0, method='fft'):
+        self._c = None
+        down = np.ascontiguousarray(down, np.int32)
+        across = np.ascontiguousarray(across, np.int32)
+        if down.ndim != 1 or across.ndim != 1:
+            raise ValueError('framecor: down and across must be sequences of integers')
+        m = _METHODS.get(method)
+        # the reference checks the parameters first, then the backend (cxx/bindings.cpp:37-44)
+        _check(_lib.framecor_check_params(int(height), int(width), _ptr(down), len(down), _ptr(across), len(across),
+                                          int(window), int(search), 0 if m is None else m, int(batch), int(stat_size)))
+        if backend not in ('auto', 'hip'):
+            raise ValueError(f"framecor: the backend must be auto or hip, not {backend}")
+        if m is None:
+            raise ValueError(f"framecor: the method must be fft or int8, not {method}")
+        int(threads)
+        handle = ctypes.c_void_p()
+        _check(_lib.framecor_create(int(height), int(width), _ptr(down), len(down), _ptr(across), len(across),
+                                    int(window), int(search), m, int(batch), int(stat_size), int(device),
+                                    ctypes.byref(handle)))
+        self._c = handle
+        self._h, self._w = int(height), int(width)
+        self._window, self._search = int(window), int(search)
+        self._shape = (len(down), len(across))
+        self._method = method
+        self._device = int(device)
+        self._ready = False
+
+    def __del__(self):
+        c = getattr(self, '_c', None)
+        if c:
+            _lib.framecor_destroy(c)
+            self._c = None
+
+    @property
+    def backend(self):
+        return _lib.framecor_backend(self._c).decode()
+
+    @property
+    def shape(self):
+        """the number of windows (down, across)"""
+        return self._shape
+
+    @property
+    def window(self):
+        return self._window
+
+    @property
+    def search(self):
+        return self._search
+
+    @property
+    def method(self):
+        return self._method
+
+    def set_reference(self, frame):
+        """Set the reference frame (height x width, uint8)"""
+        if _is_device_tensor(frame):
+            frame = frame.cpu().numpy()
+        frame = np.ascontiguousarray(frame, np.uint8)
+        if frame.ndim != 2 or frame.shape != (self._h, self._w):
+            raise ValueError('framecor: the reference frame must be height x width')
+        _check(_lib.framecor_set_reference(self._c, _ptr(frame)))
+        self._ready = True
+
+    def set_timing(self, enable=True):
+        """Record the GPU time of each stage of the next correlate calls (hipEvents on their stream)"""
+        _check(_lib.framecor_set_timing(self._c, int(bool(enable))))
+
+    def timing(self):
+        """The GPU milliseconds of the stages of the last correlate call: a dict with 'summed_area',
+        'box_sums', 'correlate' and 'launches' (waits for that call to finish)"""
+        ms = (ctypes.c_float * 3)()
+        launches = ctypes.c_int()
+        _check(_lib.framecor_get_timing(self._c, ms, ctypes.byref(launches)))
+        return dict(summed_area=ms[0], box_sums=ms[1], correlate=ms[2], launches=launches.value)
+
+    def _gross(self, gross, n, module):
+        ny, nx = self._shape
+        g = gross
+        if g.ndim == 2 and g.shape[0] == n and g.shape[1] == 2:
+            return module.broadcast(g, n, ny, nx)
+        if g.ndim == 4 and tuple(g.shape) == (n, ny, nx, 2):
+            return g
+        raise ValueError('framecor: gross must be frames x 2 or frames x down x across x 2')
+
+    def correlate(self, frames, gross=None):
+        """
+        Correlate the frames (frames x height x width, uint8) with the reference frame
+
+        gross: integer offsets (down, across) of the search windows, frames x 2 or
+        frames x down x across x 2 (default: none)
+
+        Returns a dict of float32 arrays: offsets (frames x down x across x 2: down, across, the motion
+        of the frame; NaN if not valid), peak (the normalized correlation), snr, cov (the covariance of
+        the offsets: down, across, cross). With a torch tensor on the GPU, torch tensors on that GPU,
+        computed on the current stream.
+        """
+        if not self._ready:
+            raise RuntimeError('framecor: set_reference first')
+        if _is_device_tensor(frames):
+            return self._correlate_device(frames, gross)
+        frames = np.ascontiguousarray(frames, np.uint8)
+        if frames.ndim != 3 or frames.shape[1] != self._h or frames.shape[2] != self._w:
+            raise ValueError('framecor: the frames must be frames x height x width')
+        n = frames.shape[0]
+        ny, nx = self._shape
+        g = None
+        if gross is not None:
+            g = np.ascontiguousarray(self._gross(np.asarray(gross), n, _NumpyOps), np.int32)
+        out = dict(offsets=np.empty((n, ny, nx, 2), np.float32), peak=np.empty((n, ny, nx), np.float32),
+                   snr=np.empty((n, ny, nx), np.float32), cov=np.empty((n, ny, nx, 3), np.float32))
+        if n > 0:
+            _check(_lib.framecor_correlate(self._c, _ptr(frames), n, _ptr(g), _ptr(out['offsets']),
+                                           _ptr(out['peak']), _ptr(out['snr']), _ptr(out['cov'])))
+        return out
+
+    def _correlate_device(self, frames, gross):
+        import torch
+        if frames.dtype != torch.uint8 or frames.dim() != 3 or tuple(frames.shape[1:]) != (self._h, self._w):
+            raise ValueError('framecor: the frames must be frames x height x width (uint8)')
+        if frames.device.index != self._device:
+            raise ValueError(f'framecor: the frames are on {frames.device}, the correlator on GPU {self._device}')
+        frames = frames.contiguous()
+        n = frames.shape[0]
+        ny, nx = self._shape
+        g = None
+        if gross is not None:
+            if not isinstance(gross, torch.Tensor):
+                gross = torch.as_tensor(np.asarray(gross), device=frames.device)
+            g = self._gross(gross.to(device=frames.device, dtype=torch.int32), n, _TorchOps).contiguous()
+        kw = dict(dtype=torch.float32, device=frames.device)
+        out = dict(offsets=torch.empty((n, ny, nx, 2), **kw), peak=torch.empty((n, ny, nx), **kw),
+                   snr=torch.empty((n, ny, nx), **kw), cov=torch.empty((n, ny, nx, 3), **kw))
+        if n > 0:
+            stream = torch.cuda.current_stream(frames.device).cuda_stream
+            _check(_lib.framecor_correlate_device(
+                self._c, frames.data_ptr(), n, None if g is None else g.data_ptr(), out['offsets'].data_ptr(),
+                out['peak'].data_ptr(), out['snr'].data_ptr(), out['cov'].data_ptr(), stream))
+        return out
+
+
+class _NumpyOps:
+    @staticmethod
+    def broadcast(g, n, ny, nx):
+        return np.broadcast_to(g.astype(np.int32)[:, None, None, :], (n, ny, nx, 2))
+
+
+class _TorchOps:
+    @staticmethod
+    def broadcast(g, n, ny, nx):
+        return g[:, None, None, :].expand(n, ny, nx, 2)

@@ -1,0 +1,100 @@
+"""
+Seeded synthetic frames for the framecor workloads (numpy/scipy; no downloads)
+
+  texture      white noise N(0, 1), Gaussian-filtered (sigma 1.5 px), rescaled linearly to 0-255
+  small_shifts 256 x 320 reference, 8 frames: the reference bilinearly shifted by a per-frame
+               (down, across) offset uniform in +-3 px, plus Gaussian noise sigma 2 DN, rounded and
+               clipped to uint8; window 32, search 16, skip 32 (48 windows)
+  fault_video  720 x 1280, 64 frames: a straight fault at 30 degrees through the image centre; the
+               whole frame shakes by a random walk (step sigma 0.5 px, bounded +-5 px) and the far side
+               of the fault adds a sigmoid slip ramp to 40 px along the fault over frames 20-40
+               (centre 30, scale 2.5 frames); two fixed text-like boxes of 0/255 blocks (370 x 45 at
+               x 870-1240, y 25-70 and 590 x 50 at x 40-630, y 650-700); noise sigma 2 DN; window 64,
+               search 48, skip 16 (2450 windows)
+  gross_offsets per-window integer gross offsets uniform in -20..20
+
+Seeds: texture 0, shifts 1, noise 2, gross and clip parameters 3, boxes 4.
+"""
+
+import numpy as np
+from scipy import ndimage
+
+
+def texture(height, width, sigma=1.5, seed=0):
+    """Smoothed white noise rescaled to 0-255 (float64)"""
+    img = ndimage.gaussian_filter(np.random.default_rng(seed).normal(size=(height, width)), sigma)
+    return (img - img.min()) / (img.max() - img.min()) * 255.0
+
+
+def to_uint8(img):
+    return np.clip(np.rint(img), 0, 255).astype(np.uint8)
+
+
+def warp(image, dy, dx):
+    """The image moved by the displacement (dy, dx) (scalars or per-pixel arrays): out(y, x) =
+    image(y - dy, x - dx), bilinear, reflected at the borders"""
+    h, w = image.shape
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    return ndimage.map_coordinates(image, [yy - dy, xx - dx], order=1, mode='reflect')
+
+
+def small_shifts(n=8, height=256, width=320):
+    """The CPU-sized workload: a dict with ref, frames, shifts (n x 2) and the grid parameters"""
+    ref = to_uint8(texture(height, width))
+    shifts = np.random.default_rng(1).uniform(-3, 3, size=(n, 2))
+    noise = np.random.default_rng(2)
+    base = ref.astype(np.float64)
+    frames = np.stack([to_uint8(warp(base, s[0], s[1]) + noise.normal(0.0, 2.0, base.shape)) for s in shifts])
+    return dict(ref=ref, frames=frames, shifts=shifts, window=32, search=16, skip=32)
+
+
+def text_boxes(height, width, seed=4):
+    """The mask and the pixel values of the two fixed text-like boxes (x0, y0, x1, y1)"""
+    boxes = [(870, 25, 1240, 70), (40, 650, 630, 700)]
+    rng = np.random.default_rng(seed)
+    mask = np.zeros((height, width), bool)
+    values = np.zeros((height, width), np.uint8)
+    for x0, y0, x1, y1 in boxes:
+        x1, y1 = min(x1, width), min(y1, height)
+        if x0 >= x1 or y0 >= y1:
+            continue
+        cells = rng.integers(0, 2, size=((y1 - y0 + 6) // 7, (x1 - x0 + 4) // 5)).astype(np.uint8) * 255
+        block = np.kron(cells, np.ones((7, 5), np.uint8))[:y1 - y0, :x1 - x0]
+        mask[y0:y1, x0:x1] = True
+        values[y0:y1, x0:x1] = block
+    return mask, values, boxes
+
+
+def fault_video(n=64, height=720, width=1280, angle=30.0, slip=40.0, centre=30.0, scale=2.5):
+    """The README-shaped workload: a dict with ref, frames, shake (n x 2), slip (n), side (h x w bool),
+    boxes and the grid parameters"""
+    base = texture(height, width)
+    mask, values, boxes = text_boxes(height, width)
+    ref = to_uint8(base)
+    ref[mask] = values[mask]
+    rng = np.random.default_rng(3)
+    shake = np.zeros((n, 2))
+    for f in range(1, n):
+        shake[f] = np.clip(shake[f - 1] + rng.normal(0.0, 0.5, 2), -5.0, 5.0)
+    a = np.deg2rad(angle)
+    along = np.array([-np.sin(a), np.cos(a)])       # (down, across) direction of the fault
+    normal = np.array([np.cos(a), np.sin(a)])
+    yy, xx = np.mgrid[0:height, 0:width]
+    side = (yy - height / 2) * normal[0] + (xx - width / 2) * normal[1] > 0
+    ramp = slip / (1.0 + np.exp(-(np.arange(n) - centre) / scale))
+    noise = np.random.default_rng(2)
+    src = base.copy()
+    frames = np.empty((n, height, width), np.uint8)
+    for f in range(n):
+        dy = shake[f, 0] + side * ramp[f] * along[0]
+        dx = shake[f, 1] + side * ramp[f] * along[1]
+        frame = to_uint8(warp(src, dy, dx) + noise.normal(0.0, 2.0, src.shape))
+        frame[mask] = values[mask]
+        frames[f] = frame
+    return dict(ref=ref, frames=frames, shake=shake, slip=ramp, along=along, side=side, boxes=boxes,
+                window=64, search=48, skip=16)
+
+
+def gross_offsets(n, ny, nx, low=-20, high=20, seed=3):
+    """Per-window integer gross offsets, n x ny x nx x 2 int32, uniform in low..high"""
+    return np.random.default_rng(seed).integers(low, high + 1, size=(n, ny, nx, 2)).astype(np.int32)

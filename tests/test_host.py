@@ -1,0 +1,110 @@
+"""
+Host-side checks that need no GPU: the C ABI library loads and exports every entry point that
+include/framecor_hip.h declares, window_grid, and the argument checks of the Correlator (which the
+library makes before touching a GPU, with the reference's messages, cxx/common.cpp:15-28).
+"""
+
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from slipcam_amd import framecor
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for header in glob.glob(os.path.join(ROOT, 'include', '*.h')):
+        text = open(header).read()
+        text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+        names |= set(re.findall(r'\b(framecor_\w+)\s*\(', text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(framecor._LIB_PATH)
+    names = declared_symbols()
+    assert 'framecor_create' in names and 'framecor_correlate_device' in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version_and_probe():
+    assert framecor.__version__ == '0.1.0'
+    assert isinstance(framecor.has_gpu(), bool)
+
+
+def test_window_grid_readme_shape():
+    down, across = framecor.window_grid(720, 1280, 64, 48, 16)
+    assert len(down) * len(across) == 2450           # README.md:88, 96 of the reference
+    assert down[0] == 48 and across[0] == 48 and down[-1] == 592 and across[-1] == 1152
+
+
+@pytest.mark.parametrize('h,w,W,R,skip,margin', [(240, 320, 32, 24, 24, 0), (256, 320, 32, 16, 32, 0),
+                                                 (240, 320, 32, 6, 16, 32), (1080, 1920, 128, 64, 32, 0)])
+def test_window_grid_search_windows_in_frame(h, w, W, R, skip, margin):
+    down, across = framecor.window_grid(h, w, W, R, skip, margin=margin)
+    assert down.dtype.kind == 'i'
+    for y in down:
+        assert y - R - margin >= 0 and y + W + R + margin <= h
+    for x in across:
+        assert x - R - margin >= 0 and x + W + R + margin <= w
+    # one more window would not fit
+    assert down[-1] + skip + W + R + margin > h - skip or len(down) == (h - 2 * (margin + R) - W) // skip
+
+
+def test_window_grid_nothing_fits():
+    with pytest.raises(ValueError, match='no window fits down'):
+        framecor.window_grid(100, 1000, 64, 48, 16)
+    with pytest.raises(ValueError, match='no window fits across'):
+        framecor.window_grid(1000, 100, 64, 48, 16)
+
+
+def test_arguments_without_gpu():
+    with pytest.raises(ValueError, match='outside the frame'):
+        framecor.Correlator(240, 320, [0], [300], window=32, search=8, backend='hip')
+    with pytest.raises(ValueError, match='outside the frame'):
+        framecor.Correlator(240, 320, [0], [300], window=32, search=8, backend='gpu')   # parameters first
+    for backend in ('gpu', 'cuda', 'cpu'):
+        with pytest.raises(ValueError, match='backend'):
+            framecor.Correlator(240, 320, [0], [0], window=32, search=8, backend=backend)
+    with pytest.raises(ValueError, match='method'):
+        framecor.Correlator(240, 320, [0], [0], window=32, search=8, method='fp16')
+    with pytest.raises(ValueError, match='at least 3 pixels'):
+        framecor.Correlator(240, 320, [0], [0], window=2, search=8)
+    with pytest.raises(ValueError, match='half search range'):
+        framecor.Correlator(240, 320, [0], [0], window=32, search=0)
+    with pytest.raises(ValueError, match='SNR window'):
+        framecor.Correlator(240, 320, [0], [0], window=32, search=8, stat_size=2)
+    with pytest.raises(ValueError, match='batch'):
+        framecor.Correlator(240, 320, [0], [0], window=32, search=8, batch=0)
+    with pytest.raises(ValueError, match='no windows'):
+        framecor.Correlator(240, 320, [], [0], window=32, search=8)
+    with pytest.raises(ValueError, match='frame size'):
+        framecor.Correlator(0, 320, [0], [0], window=32, search=8)
+    # what the methods support
+    with pytest.raises(ValueError, match='int8'):
+        framecor.Correlator(1000, 1000, [0], [0], window=160, search=8, method='int8')
+    with pytest.raises(ValueError, match='fft'):
+        framecor.Correlator(1000, 1000, [0], [0], window=3, search=22, method='fft')     # L = 47, prime
+    with pytest.raises(ValueError, match='fft'):
+        framecor.Correlator(1000, 1000, [0], [0], window=128, search=64, method='fft')   # L = 256 > 192
+
+
+@pytest.mark.skipif(framecor.has_gpu(), reason='a GPU is present')
+def test_no_gpu_is_a_runtime_error():
+    with pytest.raises(RuntimeError, match='no GPU'):
+        framecor.Correlator(240, 320, [0], [0], window=32, search=8)
+
+
+def test_product_never_uses_the_oracle():
+    """The package neither imports nor loads anything from oracle/ (the checker)"""
+    for path in glob.glob(os.path.join(ROOT, 'slipcam_amd', '**', '*.*'), recursive=True):
+        if path.endswith(('.py', '.hip', '.h')):
+            text = open(path).read()
+            assert 'import oracle' not in text and 'from oracle' not in text and 'liboracle' not in text, path
