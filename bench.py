@@ -103,6 +103,9 @@ def main():
     p.add_argument('--frames', type=int, default=64)
     p.add_argument('--cpu-seconds', type=float, default=15.0)
     p.add_argument('--no-cpu-baseline', action='store_true')
+    p.add_argument('--methods', default='int8,fft', help='the forms to time (the first is the headline)')
+    p.add_argument('--copy-bytes', type=int, default=0,
+                   help='instead: time a plain device copy of this many bytes (calibrates the HBM counters)')
     args = p.parse_args()
     if args.gpus != 1:
         print('bench.py: this build measures one GPU only (the correlator is single-device; frames or '
@@ -114,15 +117,33 @@ def main():
 
     from slipcam_amd import framecor, synthetic
 
+    if args.copy_bytes:
+        src = torch.randint(0, 255, (args.copy_bytes,), dtype=torch.uint8, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(args.warmup):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        print(json.dumps({'copy_bytes': args.copy_bytes, 'ms': dt * 1e3, 'GB/s': 2 * args.copy_bytes / dt / 1e9}))
+        return
+
+    methods = args.methods.split(',')
     log('bench: generating the workload')
     d = synthetic.fault_video(n=args.frames)
     dev = torch.from_numpy(d['frames']).cuda()
     torch.cuda.synchronize()
     res = {}
-    for method in ('int8', 'fft'):
+    for method in methods:
         log(f'bench: timing {method}')
         res[method] = time_method(torch, framecor, d, dev, method, args.steps, args.warmup)
         log(f'bench: {method} {res[method]["ms_per_frame"]:.4f} ms/frame')
+    if methods != ['int8', 'fft']:
+        print(json.dumps({m: {k: v for k, v in r.items() if k != 'cor'} for m, r in res.items()}), flush=True)
+        return
     i8, ff = res['int8'], res['fft']
     W, R, nwin = d['window'], d['search'], i8['nwin']
     M = 2 * R + 1

@@ -97,6 +97,24 @@ int framecor_correlate(framecor_correlator *c, const uint8_t *frames, int n, con
 int framecor_correlate_device(framecor_correlator *c, const uint8_t *frames, int n, const int32_t *gross,
                               float *offset, float *peak, float *snr, float *cov, void *stream);
 
+/* The differentiable form (no counterpart in the reference). For n float32 frames (device memory),
+ * the windows of the correlator's reference and a continuous shift per window, shifts n x nwin x 2
+ * (down, across, relative to the window plus its gross offset; gross as in framecor_correlate, or
+ * NULL), the chip of each window is sampled bilinearly and compared with the reference window:
+ *   framecor_ncc_forward   ncc[q], the NCC (0 where the chip leaves the frame or is flat)
+ *   framecor_ncc_backward  given grad_ncc[q]: adds dL/dframes into grad_frames (n x h x w) and writes
+ *                          dL/dshifts into grad_shifts (n x nwin x 2)
+ *   framecor_ncc_refine    steps Gauss-Newton steps from shifts_in: shifts_out and their ncc (NaN / 0
+ *                          where the chip leaves the frame)
+ * All arrays in device memory, enqueued on stream (NULL: the correlator's stream), not synchronized. */
+int framecor_ncc_forward(framecor_correlator *c, const float *frames, int n, const int32_t *gross,
+                         const float *shifts, float *ncc, void *stream);
+int framecor_ncc_backward(framecor_correlator *c, const float *frames, int n, const int32_t *gross,
+                          const float *shifts, const float *grad_ncc, float *grad_frames, float *grad_shifts,
+                          void *stream);
+int framecor_ncc_refine(framecor_correlator *c, const float *frames, int n, const int32_t *gross,
+                        const float *shifts_in, int steps, float *shifts_out, float *ncc, void *stream);
+
 /* per-stage GPU timing (no counterpart in the reference; for benchmarks): with enable 1, the
  * correlate calls record hipEvents on their stream around each stage; framecor_get_timing waits for
  * the last call's events and writes the milliseconds of its stages, summed over its groups of frames:

@@ -277,6 +277,22 @@ struct framecor_correlator {
         }
     }
 
+    DiffArgs diffArgs(const float *frames, int n, const int32_t *gross, const float *shifts, float *ncc) const
+    {
+        if (!ready) throw std::runtime_error("framecor: set_reference first");
+        if (n < 0) invalid("the number of frames must not be negative");
+        DiffArgs d{};
+        d.g = g;
+        d.wd = windowData();
+        d.tmpl = dTmpl.as<uint8_t>();
+        d.frames = frames;
+        d.n = n;
+        d.gross = gross;
+        d.shifts = shifts;
+        d.ncc = ncc;
+        return d;
+    }
+
     void setReference(const uint8_t *frame)
     {
         FC_HIP_CHECK(hipSetDevice(device));
@@ -510,6 +526,50 @@ int framecor_correlate_device(framecor_correlator *c, const uint8_t *frames, int
     return guarded([&] {
         c->correlateDevice(frames, n, gross, offset, peak, snr, cov,
                            stream ? static_cast<hipStream_t>(stream) : c->stream);
+    });
+}
+
+int framecor_ncc_forward(framecor_correlator *c, const float *frames, int n, const int32_t *gross,
+                         const float *shifts, float *ncc, void *stream)
+{
+    if (!c) return fail(FRAMECOR_EINVAL, "framecor: null argument");
+    if (n > 0 && (!frames || !shifts || !ncc)) return fail(FRAMECOR_EINVAL, "framecor: null argument");
+    return guarded([&] {
+        DiffArgs d = c->diffArgs(frames, n, gross, shifts, ncc);
+        FC_HIP_CHECK(hipSetDevice(c->device));
+        launchDiff(d, DiffOp::Forward, stream ? static_cast<hipStream_t>(stream) : c->stream);
+    });
+}
+
+int framecor_ncc_backward(framecor_correlator *c, const float *frames, int n, const int32_t *gross,
+                          const float *shifts, const float *grad_ncc, float *grad_frames, float *grad_shifts,
+                          void *stream)
+{
+    if (!c) return fail(FRAMECOR_EINVAL, "framecor: null argument");
+    if (n > 0 && (!frames || !shifts || !grad_ncc || !grad_frames || !grad_shifts))
+        return fail(FRAMECOR_EINVAL, "framecor: null argument");
+    return guarded([&] {
+        DiffArgs d = c->diffArgs(frames, n, gross, shifts, nullptr);
+        d.gradNcc = grad_ncc;
+        d.gradFrames = grad_frames;
+        d.gradShifts = grad_shifts;
+        FC_HIP_CHECK(hipSetDevice(c->device));
+        launchDiff(d, DiffOp::Backward, stream ? static_cast<hipStream_t>(stream) : c->stream);
+    });
+}
+
+int framecor_ncc_refine(framecor_correlator *c, const float *frames, int n, const int32_t *gross,
+                        const float *shifts_in, int steps, float *shifts_out, float *ncc, void *stream)
+{
+    if (!c) return fail(FRAMECOR_EINVAL, "framecor: null argument");
+    if (n > 0 && (!frames || !shifts_in || !shifts_out || !ncc)) return fail(FRAMECOR_EINVAL, "framecor: null argument");
+    if (steps < 0 || steps > 100) return fail(FRAMECOR_EINVAL, "framecor: steps must be in 0..100");
+    return guarded([&] {
+        DiffArgs d = c->diffArgs(frames, n, gross, shifts_in, ncc);
+        d.shiftsOut = shifts_out;
+        d.steps = steps;
+        FC_HIP_CHECK(hipSetDevice(c->device));
+        launchDiff(d, DiffOp::Refine, stream ? static_cast<hipStream_t>(stream) : c->stream);
     });
 }
 

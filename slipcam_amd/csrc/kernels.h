@@ -69,4 +69,26 @@ void launchFftReference(const FftArgs &a, hipStream_t stream);
 // correlate a group of frames
 void launchFftCorrelate(FftArgs a, hipStream_t stream);
 
+// ---- the differentiable form (diff.hip)
+
+struct DiffArgs {
+    Geometry g;
+    WindowData wd;
+    const uint8_t *tmpl;      // nwin x W x W, the reference windows
+    const float *frames;      // n x h x w float32
+    int n;
+    const int *gross;         // n x nwin x 2, or null
+    const float *shifts;      // n x nwin x 2: (sy, sx), continuous, relative to the window + gross
+    float *ncc;               // n x nwin
+    const float *gradNcc;     // backward: n x nwin
+    float *gradFrames;        // backward: n x h x w, accumulated
+    float *gradShifts;        // backward: n x nwin x 2
+    float *shiftsOut;         // refine: n x nwin x 2
+    int steps;                // refine: Gauss-Newton steps
+};
+
+enum class DiffOp { Forward, Backward, Refine };
+
+void launchDiff(DiffArgs d, DiffOp op, hipStream_t stream);
+
 } // namespace framecor

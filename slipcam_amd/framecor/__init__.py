@@ -29,6 +29,14 @@ _METHODS = {'fft': 0, 'int8': 1}
 
 
 def _load():
+    # One HIP runtime per process: torch wheels bundle their own libamdhip64 (soname libamdhip64.so.7),
+    # which this library's dependency resolves to when torch is loaded first; loaded the other way
+    # round the process would map two runtimes, and torch tensors and streams handed to the library
+    # would belong to another runtime. So torch, when installed, is loaded first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(_LIB_PATH):
         raise ImportError(f'framecor: {_LIB_PATH} is not built (python -m slipcam_amd.build)')
     lib = ctypes.CDLL(_LIB_PATH)
@@ -50,6 +58,12 @@ def _load():
     lib.framecor_correlate.argtypes = [P, P, I, P, P, P, P, P]
     lib.framecor_correlate_device.restype = I
     lib.framecor_correlate_device.argtypes = [P, P, I, P, P, P, P, P, P]
+    lib.framecor_ncc_forward.restype = I
+    lib.framecor_ncc_forward.argtypes = [P, P, I, P, P, P, P]
+    lib.framecor_ncc_backward.restype = I
+    lib.framecor_ncc_backward.argtypes = [P, P, I, P, P, P, P, P, P]
+    lib.framecor_ncc_refine.restype = I
+    lib.framecor_ncc_refine.argtypes = [P, P, I, P, P, I, P, P, P]
     lib.framecor_set_timing.restype = I
     lib.framecor_set_timing.argtypes = [P, I]
     lib.framecor_get_timing.restype = I

@@ -12,6 +12,10 @@ Seeded synthetic frames for the framecor workloads (numpy/scipy; no downloads)
                x 870-1240, y 25-70 and 590 x 50 at x 40-630, y 650-700); noise sigma 2 DN; window 64,
                search 48, skip 16 (2450 windows)
   gross_offsets per-window integer gross offsets uniform in -20..20
+  smooth_shifts 1080 x 1920, 128 frames, window 64, search 32, skip 32: each frame moved by an integer
+               offset uniform in -20..20 (the gross offset of all its windows) plus a smooth sub-pixel
+               displacement field within +-0.5 px (long-wavelength products of sines); windows overlap
+               (skip 32 < window 64), so the truth of a window is the mean of the field over its pixels
 
 Seeds: texture 0, shifts 1, noise 2, gross and clip parameters 3, boxes 4.
 """
@@ -98,3 +102,32 @@ def fault_video(n=64, height=720, width=1280, angle=30.0, slip=40.0, centre=30.0
 def gross_offsets(n, ny, nx, low=-20, high=20, seed=3):
     """Per-window integer gross offsets, n x ny x nx x 2 int32, uniform in low..high"""
     return np.random.default_rng(seed).integers(low, high + 1, size=(n, ny, nx, 2)).astype(np.int32)
+
+
+def smooth_shifts(n=128, height=1080, width=1920, window=64, search=32, skip=32, seed=6):
+    """The differentiation workload: a dict with ref, frames, gross (n x 2 int32), truth (n x ny x nx x 2:
+    the gross offset plus the mean of the field over each window) and the grid parameters"""
+    base = texture(height, width)
+    ref = to_uint8(base)
+    rng = np.random.default_rng(seed)
+    gross = rng.integers(-20, 21, size=(n, 2)).astype(np.int32)
+    yy, xx = np.mgrid[0:height, 0:width].astype(np.float64)
+    noise = np.random.default_rng(2)
+    first = search
+    down = first + skip * np.arange((height - 2 * search - window) // skip)
+    across = first + skip * np.arange((width - 2 * search - window) // skip)
+    frames = np.empty((n, height, width), np.uint8)
+    truth = np.empty((n, len(down), len(across), 2))
+    for f in range(n):
+        field = []
+        for axis in range(2):
+            ly, lx = rng.uniform(500, 900, 2)
+            py, px = rng.uniform(0, 2 * np.pi, 2)
+            field.append(0.5 * np.sin(2 * np.pi * yy / ly + py) * np.sin(2 * np.pi * xx / lx + px))
+        dy, dx = gross[f, 0] + field[0], gross[f, 1] + field[1]
+        frames[f] = to_uint8(warp(base, dy, dx) + noise.normal(0.0, 2.0, base.shape))
+        for a in range(2):
+            c = np.cumsum(np.cumsum(np.pad(field[a], ((1, 0), (1, 0))), 0), 1)
+            box = c[window:, window:] - c[:-window, window:] - c[window:, :-window] + c[:-window, :-window]
+            truth[f, :, :, a] = gross[f, a] + box[np.ix_(down, across)] / (window * window)
+    return dict(ref=ref, frames=frames, gross=gross, truth=truth, window=window, search=search, skip=skip)
