@@ -22,6 +22,21 @@ namespace framecor {
 
 constexpr int kMaxRadix = 32;
 
+// floor(x / d) for 0 <= x < 2^20 and 1 <= d < 2^12 as a multiply-high by m = ceil(2^32 / d): the error
+// of x m / 2^32 against x / d is below x / 2^32 < 1 / d, which cannot carry it past the next integer
+__host__ __device__ inline unsigned magicOf(unsigned d) { return unsigned((0x100000000ull + d - 1) / d); }
+
+__host__ __device__ inline int divMagic(int x, unsigned m, int d)
+{
+    // d == 1: m would be 2^32
+    if (d == 1) return x;
+#ifdef __HIP_DEVICE_COMPILE__
+    return int(__umulhi(unsigned(x), m));
+#else
+    return int((unsigned long long)unsigned(x) * m >> 32);
+#endif
+}
+
 __host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __host__ __device__ inline float2 cmul(float2 a, float2 b)
@@ -147,9 +162,10 @@ __host__ __device__ inline void radixPass(float2 *Z, const float2 *tw, int L, in
 {
     const int Sj = Lj / R, per = L / R, tstep = L / Lj;
     const int items = nlines * per;
+    const unsigned mPer = magicOf(per), mSj = magicOf(Sj);
     for (int it = tid; it < items; it += nt) {
-        const int line = it / per, w = it - line * per;
-        const int b = w / Sj, n1 = w - b * Sj;
+        const int line = divMagic(it, mPer, per), w = it - line * per;
+        const int b = divMagic(w, mSj, Sj), n1 = w - b * Sj;
         const int e0 = b * Lj + n1;
         float2 v[R];
 #pragma unroll
@@ -176,9 +192,10 @@ __host__ __device__ inline void genericPass(int R, float2 *Z, const float2 *tw, 
 {
     const int Sj = Lj / R, per = L / R, tstep = L / Lj, rstep = L / R;
     const int items = nlines * per;
+    const unsigned mPer = magicOf(per), mSj = magicOf(Sj);
     for (int it = tid; it < items; it += nt) {
-        const int line = it / per, w = it - line * per;
-        const int b = w / Sj, n1 = w - b * Sj;
+        const int line = divMagic(it, mPer, per), w = it - line * per;
+        const int b = divMagic(w, mSj, Sj), n1 = w - b * Sj;
         const int e0 = b * Lj + n1;
         float2 v[kMaxRadix], o[kMaxRadix];
         for (int m = 0; m < R; m++) v[m] = Z[addr(line, e0 + Sj * m)];
@@ -260,12 +277,13 @@ __host__ __device__ inline void splitPairs(float2 *Z, const int *pos, int L, int
     for (int p0 = 0; p0 < np; p0 += pairsPerRound) {
         const int p1 = p0 + pairsPerRound < np ? p0 + pairsPerRound : np;
         const int items = (p1 - p0) * H;
+        const unsigned mH = magicOf(H);
         float2 A[IPT], B[IPT];
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
             const int it = tid + i * nt;
             if (it < items) {
-                const int p = p0 + it / H, k = it % H;
+                const int pp = divMagic(it, mH, H), p = p0 + pp, k = it - pp * H;
                 const float2 zk = Z[p * rs + pos[k]];
                 const float2 zc = Z[p * rs + pos[(L - k) % L]];
                 // conj(Z(L-k)) = (zc.x, -zc.y)
@@ -278,7 +296,7 @@ __host__ __device__ inline void splitPairs(float2 *Z, const int *pos, int L, int
         for (int i = 0; i < IPT; i++) {
             const int it = tid + i * nt;
             if (it < items) {
-                const int p = p0 + it / H, k = it % H;
+                const int pp = divMagic(it, mH, H), p = p0 + pp, k = it - pp * H;
                 Z[p * rs + k] = A[i];
                 Z[p * rs + H + k] = B[i];
             }
@@ -299,12 +317,13 @@ __host__ __device__ inline void combinePairs(float2 *Z, const int *pos, int L, i
     for (int p0 = 0; p0 < np; p0 += pairsPerRound) {
         const int p1 = p0 + pairsPerRound < np ? p0 + pairsPerRound : np;
         const int items = (p1 - p0) * L;
+        const unsigned mL = magicOf(L);
         float2 z[IPT];
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
             const int it = tid + i * nt;
             if (it < items) {
-                const int p = p0 + it / L, k = it % L;
+                const int pp = divMagic(it, mL, L), p = p0 + pp, k = it - pp * L;
                 const bool hasB = 2 * p + 1 < L;
                 float2 a, b = make_float2(0.0f, 0.0f);
                 if (k < H) {
@@ -331,7 +350,7 @@ __host__ __device__ inline void combinePairs(float2 *Z, const int *pos, int L, i
         for (int i = 0; i < IPT; i++) {
             const int it = tid + i * nt;
             if (it < items) {
-                const int p = p0 + it / L, k = it % L;
+                const int pp = divMagic(it, mL, L), p = p0 + pp, k = it - pp * L;
                 Z[p * rs + pos[k]] = z[i];
             }
         }

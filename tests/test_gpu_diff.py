@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from slipcam_amd import framecor, synthetic
+from slipcam_amd.framecor import autograd
 
 pytestmark = pytest.mark.gpu
 
@@ -63,7 +64,7 @@ def test_forward_matches_fp64(gpu, small):
     d, down, across, shifts, gross, cor = small
     frames = torch.tensor(d['frames'], dtype=torch.float32, device='cuda')
     s = torch.tensor(shifts, dtype=torch.float32, device='cuda')
-    got = framecor.autograd.ncc_shift(cor, frames, s, gross=gross).cpu().numpy()
+    got = autograd.ncc_shift(cor, frames, s, gross=gross).cpu().numpy()
     ref = ncc_reference(torch.tensor(d['frames'], dtype=torch.float64), d['ref'], down, across, 32, gross,
                         torch.tensor(s.cpu().numpy(), dtype=torch.float64)).numpy()
     assert np.abs(got - ref).max() < 2e-6
@@ -74,7 +75,7 @@ def test_backward_matches_fp64_autograd(gpu, small):
     frames = torch.tensor(d['frames'], dtype=torch.float32, device='cuda', requires_grad=True)
     s = torch.tensor(shifts, dtype=torch.float32, device='cuda', requires_grad=True)
     weights = torch.tensor(np.random.default_rng(8).normal(size=shifts.shape[:-1]), dtype=torch.float32)
-    ncc = framecor.autograd.ncc_shift(cor, frames, s, gross=gross)
+    ncc = autograd.ncc_shift(cor, frames, s, gross=gross)
     (ncc * weights.cuda()).sum().backward()
     f64 = torch.tensor(d['frames'], dtype=torch.float64, requires_grad=True)
     s64 = torch.tensor(s.detach().cpu().numpy(), dtype=torch.float64, requires_grad=True)
@@ -93,7 +94,7 @@ def test_shift_gradient_central_differences(gpu, small):
     h = 1e-3
     frames = torch.tensor(d['frames'], dtype=torch.float32, device='cuda')
     s = torch.tensor(shifts, dtype=torch.float32, device='cuda', requires_grad=True)
-    framecor.autograd.ncc_shift(cor, frames, s, gross=gross).sum().backward()
+    autograd.ncc_shift(cor, frames, s, gross=gross).sum().backward()
     analytic = s.grad.cpu().numpy()
     f64 = torch.tensor(d['frames'], dtype=torch.float64)
     base = torch.tensor(s.detach().cpu().numpy(), dtype=torch.float64)
@@ -105,8 +106,8 @@ def test_shift_gradient_central_differences(gpu, small):
         assert np.abs(fd64 - analytic[..., axis]).max() < 2e-3 * max(1.0, np.abs(fd64).max())
         sp = (s.detach() + torch.tensor(e.numpy(), dtype=torch.float32, device='cuda'))
         sm = (s.detach() - torch.tensor(e.numpy(), dtype=torch.float32, device='cuda'))
-        fd32 = ((framecor.autograd.ncc_shift(cor, frames, sp, gross=gross)
-                 - framecor.autograd.ncc_shift(cor, frames, sm, gross=gross)) / (2 * h)).cpu().numpy()
+        fd32 = ((autograd.ncc_shift(cor, frames, sp, gross=gross)
+                 - autograd.ncc_shift(cor, frames, sm, gross=gross)) / (2 * h)).cpu().numpy()
         assert np.abs(fd32 - analytic[..., axis]).max() < 5e-3
 
 
@@ -125,7 +126,7 @@ def test_gauss_newton_beside_quadratic_fit(gpu):
     g4 = np.broadcast_to(gross[:, None, None, :], quad.shape)
     start = np.where(np.isfinite(quad), quad - g4, 0.0)
     frames = torch.tensor(d['frames'], dtype=torch.float32, device='cuda')
-    refined, ncc = framecor.autograd.refine(cor, frames, torch.tensor(start, dtype=torch.float32, device='cuda'),
+    refined, ncc = autograd.refine(cor, frames, torch.tensor(start, dtype=torch.float32, device='cuda'),
                                             gross=gross, steps=3)
     gn = refined.cpu().numpy() + g4
     ok = valid & np.isfinite(gn[..., 0])
