@@ -90,8 +90,8 @@ int framecor_set_reference(framecor_correlator *c, const uint8_t *frame);
 int framecor_correlate(framecor_correlator *c, const uint8_t *frames, int n, const int32_t *gross,
                        float *offset, float *peak, float *snr, float *cov);
 
-/* the same with every array in device memory, enqueued on stream (a hipStream_t; NULL: the
- * correlator's own stream) and not synchronized: the outputs are complete when the stream is. The
+/* the same with every array in device memory, enqueued on stream (a hipStream_t; NULL: the null
+ * stream) and not synchronized: the outputs are complete when the stream is. The
  * reference has no device-resident entry point; this is the one its CUDA backend's batch loop
  * (cxx/cuda.cu:329-359) runs between its copies. */
 int framecor_correlate_device(framecor_correlator *c, const uint8_t *frames, int n, const int32_t *gross,
@@ -106,7 +106,7 @@ int framecor_correlate_device(framecor_correlator *c, const uint8_t *frames, int
  *                          dL/dshifts into grad_shifts (n x nwin x 2)
  *   framecor_ncc_refine    steps Gauss-Newton steps from shifts_in: shifts_out and their ncc (NaN / 0
  *                          where the chip leaves the frame)
- * All arrays in device memory, enqueued on stream (NULL: the correlator's stream), not synchronized. */
+ * All arrays in device memory, enqueued on stream (NULL: the null stream), not synchronized. */
 int framecor_ncc_forward(framecor_correlator *c, const float *frames, int n, const int32_t *gross,
                          const float *shifts, float *ncc, void *stream);
 int framecor_ncc_backward(framecor_correlator *c, const float *frames, int n, const int32_t *gross,
@@ -114,6 +114,11 @@ int framecor_ncc_backward(framecor_correlator *c, const float *frames, int n, co
                           void *stream);
 int framecor_ncc_refine(framecor_correlator *c, const float *frames, int n, const int32_t *gross,
                         const float *shifts_in, int steps, float *shifts_out, float *ncc, void *stream);
+
+/* the kernel of the int8 method (no counterpart in the reference; a tuning knob): 0 automatic (the
+ * pipelined producer/helper kernel where its LDS buffers fit, else the one-frame-at-a-time kernel),
+ * 1 one frame at a time, 2 pipelined (falls back to 1 where it does not fit); the results are the same */
+int framecor_set_variant(framecor_correlator *c, int variant);
 
 /* per-stage GPU timing (no counterpart in the reference; for benchmarks): with enable 1, the
  * correlate calls record hipEvents on their stream around each stage; framecor_get_timing waits for

@@ -120,12 +120,13 @@ struct framecor_correlator {
     bool ready = false;
     int maxGroup = 1;
     Int8Layout i8{};
+    Int8Variant variant = Int8Variant::Auto;
     FftPlan plan{};
 
     // per window, written by set_reference
     DeviceBuffer dDown, dAcross, dNorm, dTsum, dTvar, dTmpl, dSpectra, dTwiddle;
     // per group of frames
-    DeviceBuffer dSat, dSat2, dBox, dBox2;
+    DeviceBuffer dSat, dSat2, dBox, dBox2, dRsig;
     int groupCap = 0;
     // host-call staging
     DeviceBuffer dFrames, dGross, dOffset, dPeak, dSnr, dCov;
@@ -156,7 +157,7 @@ struct framecor_correlator {
 
     size_t bytesPerFrame() const
     {
-        return size_t(g.satSize) * 16 + size_t(g.boxSize) * 8;
+        return size_t(g.satSize) * 16 + size_t(g.boxSize) * 12;
     }
 
     void ensureGroup(int n)
@@ -167,6 +168,7 @@ struct framecor_correlator {
         dSat2.ensure(size_t(want) * g.satSize * sizeof(int64_t));
         dBox.ensure(size_t(want) * g.boxSize * sizeof(int32_t));
         dBox2.ensure(size_t(want) * g.boxSize * sizeof(int32_t));
+        if (method == FRAMECOR_METHOD_INT8) dRsig.ensure(size_t(want) * g.boxSize * sizeof(float));
         groupCap = want;
     }
 
@@ -189,7 +191,7 @@ struct framecor_correlator {
         launchSummedArea(frames, n, g.h, g.w, dSat.as<int64_t>(), dSat2.as<int64_t>(), s);
         mark(s);
         launchBoxSums(dSat.as<int64_t>(), dSat2.as<int64_t>(), n, g.h, g.w, g.W, dBox.as<int32_t>(),
-                      dBox2.as<int32_t>(), s);
+                      dBox2.as<int32_t>(), method == FRAMECOR_METHOD_INT8 ? dRsig.as<float>() : nullptr, s);
         mark(s);
         FrameGroup fg;
         fg.frames = frames;
@@ -197,6 +199,7 @@ struct framecor_correlator {
         fg.sat = dSat.as<int64_t>();
         fg.box = dBox.as<int32_t>();
         fg.box2 = dBox2.as<int32_t>();
+        fg.rsig = method == FRAMECOR_METHOD_INT8 ? dRsig.as<float>() : nullptr;
         fg.n = n;
         fg.offset = offset;
         fg.peak = peak;
@@ -208,7 +211,7 @@ struct framecor_correlator {
             a.wd = windowData();
             a.fg = fg;
             a.tmpl = dTmpl.as<uint8_t>();
-            launchInt8(a, i8, s);
+            launchInt8(a, i8, variant, s);
         }
         else {
             FftArgs a{};
@@ -525,7 +528,7 @@ int framecor_correlate_device(framecor_correlator *c, const uint8_t *frames, int
     if (n > 0 && (!frames || !offset || !peak || !snr || !cov)) return fail(FRAMECOR_EINVAL, "framecor: null argument");
     return guarded([&] {
         c->correlateDevice(frames, n, gross, offset, peak, snr, cov,
-                           stream ? static_cast<hipStream_t>(stream) : c->stream);
+                           static_cast<hipStream_t>(stream));
     });
 }
 
@@ -537,7 +540,7 @@ int framecor_ncc_forward(framecor_correlator *c, const float *frames, int n, con
     return guarded([&] {
         DiffArgs d = c->diffArgs(frames, n, gross, shifts, ncc);
         FC_HIP_CHECK(hipSetDevice(c->device));
-        launchDiff(d, DiffOp::Forward, stream ? static_cast<hipStream_t>(stream) : c->stream);
+        launchDiff(d, DiffOp::Forward, static_cast<hipStream_t>(stream));
     });
 }
 
@@ -554,7 +557,7 @@ int framecor_ncc_backward(framecor_correlator *c, const float *frames, int n, co
         d.gradFrames = grad_frames;
         d.gradShifts = grad_shifts;
         FC_HIP_CHECK(hipSetDevice(c->device));
-        launchDiff(d, DiffOp::Backward, stream ? static_cast<hipStream_t>(stream) : c->stream);
+        launchDiff(d, DiffOp::Backward, static_cast<hipStream_t>(stream));
     });
 }
 
@@ -569,8 +572,16 @@ int framecor_ncc_refine(framecor_correlator *c, const float *frames, int n, cons
         d.shiftsOut = shifts_out;
         d.steps = steps;
         FC_HIP_CHECK(hipSetDevice(c->device));
-        launchDiff(d, DiffOp::Refine, stream ? static_cast<hipStream_t>(stream) : c->stream);
+        launchDiff(d, DiffOp::Refine, static_cast<hipStream_t>(stream));
     });
+}
+
+int framecor_set_variant(framecor_correlator *c, int variant)
+{
+    if (!c) return fail(FRAMECOR_EINVAL, "framecor: null argument");
+    if (variant < 0 || variant > 2) return fail(FRAMECOR_EINVAL, "framecor: the kernel variant must be 0, 1 or 2");
+    c->variant = static_cast<Int8Variant>(variant);
+    return FRAMECOR_OK;
 }
 
 int framecor_set_timing(framecor_correlator *c, int enable)

@@ -37,6 +37,8 @@ namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
+constexpr int kPipedWaves = 12;   // producers + helpers per block of the pipelined kernel
+
 struct WaveBest {
     double value;
     int index;
@@ -45,29 +47,126 @@ struct WaveBest {
 
 struct Moments {
     int64_t n;      // W^2
-    int64_t tsum;   // sum T
     int64_t tvar;   // W^2 sum T^2 - (sum T)^2
+    double nd;      // W^2
+    double tau;     // sum (T - 128) = sum T - 128 W^2
+    float rst;      // 1 / sqrt(tvar)
 };
 
-// the exact NCC in FP64 from the raw int8 sum and the box sums of the frame; 0 where the search box
-// (or the template) is flat, as cxx/surface.h:30-35
-__device__ inline double exactNcc(int raw, int fs, int fs2, const Moments &m)
+// the exact numerator W^2 sum(FT) - sum T sum F = W^2 r - tau phi, phi = sum F - 128 W^2: an integer
+// below 2^53, so the FP64 product tau phi and the fused multiply-add are exact
+__device__ inline double numerator(int raw, int fs, const Moments &m)
 {
-    const int64_t sft = int64_t(raw) + 128 * (int64_t(fs) + m.tsum) - 16384 * m.n;
-    const int64_t vf = m.n * int64_t(fs2) - int64_t(fs) * int64_t(fs);
-    if (vf <= 0 || m.tvar <= 0) return 0.0;
-    const double num = double(m.n) * double(sft) - double(m.tsum) * double(fs);
-    return num / sqrt(double(m.tvar) * double(vf));
+    const double phi = double(fs - 128 * int(m.n));
+    return fma(m.nd, double(raw), -(m.tau * phi));
 }
 
-// the same to FP32 accuracy (a relative error of a few 1e-7), to screen the argmax candidates
-__device__ inline float screenNcc(int raw, int fs, int fs2, const Moments &m)
+// the exact NCC in FP64 (the same value as the direct oracle's); 0 where the search box (or the
+// template) is flat, as cxx/surface.h:30-35
+__device__ inline double exactNcc(int raw, int fs, int fs2, const Moments &m)
 {
-    const int64_t sft = int64_t(raw) + 128 * (int64_t(fs) + m.tsum) - 16384 * m.n;
     const int64_t vf = m.n * int64_t(fs2) - int64_t(fs) * int64_t(fs);
-    if (vf <= 0 || m.tvar <= 0) return 0.0f;
-    const int64_t num = m.n * sft - m.tsum * int64_t(fs);
-    return float(num) * rsqrtf(float(m.tvar) * float(vf));
+    if (vf <= 0 || m.tvar <= 0) return 0.0;
+    return numerator(raw, fs, m) / sqrt(double(m.tvar) * double(vf));
+}
+
+// the same to FP32 accuracy (a relative error of a few 1e-7), with the frame's 1 / sqrt(vf) map
+__device__ inline float screenNcc(int raw, int fs, float rsig, const Moments &m)
+{
+    return float(numerator(raw, fs, m)) * rsig * m.rst;
+}
+
+// the raw int32 surface of one 16-row band (rows 16 band .. + 15, all NVT v tiles) of a staged chip
+template <int NVT, int NJB>
+__device__ inline void bandMfma(v4i (&acc)[NVT], const int8_t *chip, const int8_t *tmpl, int band, int lane, int ni4,
+                                int tstride, int cstride)
+{
+    const int g4 = lane >> 4, m16 = lane & 15;
+    // B fragment geometry: lane (g4, n = m16) takes template row 4 i4 + g4, bytes 16 + 16 jb - n .. + 15
+    // of the zero-padded LDS row, i.e. dwords (16 - n) / 4 + 4 jb .. + 4, funnel-shifted
+    const int bbyte = 16 - m16, bdw = bbyte >> 2, bsh = bbyte & 3;
+#pragma unroll
+    for (int v1 = 0; v1 < NVT; v1++) acc[v1] = v4i{0, 0, 0, 0};
+    const int8_t *crow = chip + (16 * band + g4 + m16) * cstride;
+    const uint32_t *trow = reinterpret_cast<const uint32_t *>(tmpl + g4 * tstride) + bdw;
+    const int cstep = 4 * cstride, tstep = tstride;   // 4 rows: bytes; dwords per 4 rows
+    for (int i4 = 0; i4 < ni4; i4++) {
+        v4i b[NJB];
+#pragma unroll
+        for (int jb = 0; jb < NJB; jb++) {
+            const uint32_t *t = trow + 4 * jb;
+            const uint32_t w0 = t[0], w1 = t[1], w2 = t[2], w3 = t[3], w4 = t[4];
+            b[jb] = v4i{int(__builtin_amdgcn_alignbyte(w1, w0, bsh)), int(__builtin_amdgcn_alignbyte(w2, w1, bsh)),
+                        int(__builtin_amdgcn_alignbyte(w3, w2, bsh)), int(__builtin_amdgcn_alignbyte(w4, w3, bsh))};
+        }
+#pragma unroll
+        for (int cb = 0; cb < NVT + NJB - 1; cb++) {
+            const v4i av = *reinterpret_cast<const v4i *>(crow + 16 * cb);
+#pragma unroll
+            for (int jb = 0; jb < NJB; jb++) {
+                const int v1 = cb - jb;
+                if (v1 >= 0 && v1 < NVT) acc[v1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b[jb], acc[v1], 0, 0, 0);
+            }
+        }
+        crow += cstep;
+        trow += tstep;
+    }
+}
+
+// the chip of a frame (L x L at (sy, sx)) as int8 (pixels ^ 0x80 = pixels - 128) into LDS, by threads
+// t0 .. t0 + nt - 1 of the block (t: this thread's index among them)
+__device__ inline void loadChip(int8_t *chip, const uint8_t *src, int w, int L, int cstride, int t, int nt)
+{
+    const int cpr = L >> 4;
+    for (int c = t; c < L * cpr; c += nt) {
+        const int r = c / cpr, cc = c - r * cpr;
+        uint4 v;
+        __builtin_memcpy(&v, src + size_t(r) * w + 16 * cc, 16);
+        v.x ^= 0x80808080u;
+        v.y ^= 0x80808080u;
+        v.z ^= 0x80808080u;
+        v.w ^= 0x80808080u;
+        *reinterpret_cast<uint4 *>(chip + r * cstride + 16 * cc) = v;
+    }
+    const int tail = L & 15;
+    if (tail)
+        for (int c = t; c < L * tail; c += nt) {
+            const int r = c / tail, j = 16 * cpr + (c - r * tail);
+            chip[r * cstride + j] = int8_t(src[size_t(r) * w + j] ^ 0x80);
+        }
+}
+
+// zero the template and chip areas (the padding stays zero), then the template as int8 rows at byte 16
+__device__ inline void loadTemplate(unsigned char *lds, int zeroBytes, int8_t *tmpl, const uint8_t *T, int W,
+                                    int tstride, int tid, int nthreads)
+{
+    for (int i = tid * 16; i < zeroBytes; i += nthreads * 16) *reinterpret_cast<v4i *>(lds + i) = v4i{0, 0, 0, 0};
+    __syncthreads();
+    for (int idx = tid; idx < W * W; idx += nthreads) {
+        const int i = idx / W, j = idx - i * W;
+        tmpl[i * tstride + 16 + j] = int8_t(T[idx] ^ 0x80);
+    }
+}
+
+// the window k of frame f: its gross offset, search window and validity (uniform over the block)
+struct FrameWindow {
+    int64_t q;
+    int gy, gx, sy, sx;
+    bool valid;
+};
+
+__device__ inline FrameWindow frameWindow(const Int8Args &a, int k, int f, int y0, int x0, float normk)
+{
+    FrameWindow fw;
+    fw.q = int64_t(f) * a.g.nwin + k;
+    fw.gy = 0;
+    fw.gx = 0;
+    if (a.fg.gross) {
+        fw.gy = a.fg.gross[2 * fw.q];
+        fw.gx = a.fg.gross[2 * fw.q + 1];
+    }
+    fw.valid = normk > 0 && searchWindow(a.g, y0, x0, fw.gy, fw.gx, fw.sy, fw.sx);
+    return fw;
 }
 
 template <int NVT, int NJB>
@@ -88,104 +187,40 @@ __global__ void __launch_bounds__(64 * NVT) int8Correlate(Int8Args a)
     double *f9 = psum + a.nut;
     int *pcnt = reinterpret_cast<int *>(f9 + 9);
 
-    // zero the template and chip areas (their padding stays zero), then the template as int8
-    {
-        const int bytes = 4 * a.ni4 * a.tstride + a.crows * a.cstride;
-        for (int i = tid * 16; i < bytes; i += nthreads * 16) *reinterpret_cast<v4i *>(lds + i) = v4i{0, 0, 0, 0};
-    }
-    __syncthreads();
-    {
-        const uint8_t *T = a.tmpl + size_t(k) * W * W;
-        for (int idx = tid; idx < W * W; idx += nthreads) {
-            const int i = idx / W, j = idx - i * W;
-            tmpl[i * a.tstride + 16 + j] = int8_t(T[idx] ^ 0x80);
-        }
-    }
+    loadTemplate(lds, 4 * a.ni4 * a.tstride + a.crows * a.cstride, tmpl, a.tmpl + size_t(k) * W * W, W, a.tstride,
+                 tid, nthreads);
     __syncthreads();
 
     const float normk = a.wd.norm[k];
     const int y0 = a.wd.down[k / g.nx], x0 = a.wd.across[k % g.nx];
     Moments mom;
     mom.n = int64_t(W) * W;
-    mom.tsum = a.wd.tsum[k];
     mom.tvar = a.wd.tvar[k];
-
+    mom.nd = double(mom.n);
+    mom.tau = double(a.wd.tsum[k] - 128 * mom.n);
+    mom.rst = mom.tvar > 0 ? float(1.0 / sqrt(double(mom.tvar))) : 0.0f;
     const int g4 = lane >> 4, m16 = lane & 15;
-    // B fragment geometry: lane (g4, n = m16) takes template row 4 i4 + g4, bytes 16 + 16 jb - n ..
-    // + 15 of the zero-padded LDS row, i.e. dwords (16 - n) / 4 + 4 jb .. + 4, funnel-shifted
-    const int bbyte = 16 - m16, bdw = bbyte >> 2, bsh = bbyte & 3;
 
     for (int f = fBegin; f < fEnd; f++) {
-        const int64_t q = int64_t(f) * g.nwin + k;
-        int gy = 0, gx = 0;
-        if (a.fg.gross) {
-            gy = a.fg.gross[2 * q];
-            gx = a.fg.gross[2 * q + 1];
-        }
-        int sy, sx;
-        if (!(normk > 0) || !searchWindow(g, y0, x0, gy, gx, sy, sx)) {
+        const FrameWindow fw = frameWindow(a, k, f, y0, x0, normk);
+        const int64_t q = fw.q;
+        const int gy = fw.gy, gx = fw.gx, sy = fw.sy, sx = fw.sx;
+        if (!fw.valid) {
             if (tid == 0) invalidateOut(q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
             continue;   // uniform over the block
         }
-
-        // the chip, pixels ^ 0x80 = pixels - 128 as int8 (the previous frame's reads ended at B3)
-        {
-            const uint8_t *src = a.fg.frames + size_t(f) * g.pixels + size_t(sy) * g.w + sx;
-            const int cpr = L >> 4;
-            for (int c = tid; c < L * cpr; c += nthreads) {
-                const int r = c / cpr, cc = c - r * cpr;
-                uint4 v;
-                __builtin_memcpy(&v, src + size_t(r) * g.w + 16 * cc, 16);
-                v.x ^= 0x80808080u;
-                v.y ^= 0x80808080u;
-                v.z ^= 0x80808080u;
-                v.w ^= 0x80808080u;
-                *reinterpret_cast<uint4 *>(chip + r * a.cstride + 16 * cc) = v;
-            }
-            const int tail = L & 15;
-            if (tail)
-                for (int c = tid; c < L * tail; c += nthreads) {
-                    const int r = c / tail, j = 16 * cpr + (c - r * tail);
-                    chip[r * a.cstride + j] = int8_t(src[size_t(r) * g.w + j] ^ 0x80);
-                }
-        }
+        // the chip (the previous frame's reads ended at B3)
+        loadChip(chip, a.fg.frames + size_t(f) * g.pixels + size_t(sy) * g.w + sx, g.w, L, a.cstride, tid, nthreads);
         __syncthreads();   // B1
 
-        // the band of this wave: rows 16 wave .. + 15 of the surface, all v tiles
         v4i acc[NVT];
-#pragma unroll
-        for (int v1 = 0; v1 < NVT; v1++) acc[v1] = v4i{0, 0, 0, 0};
-        {
-            const int8_t *crow = chip + (16 * wave + g4 + m16) * a.cstride;
-            const uint32_t *trow = reinterpret_cast<const uint32_t *>(tmpl + g4 * a.tstride) + bdw;
-            const int cstep = 4 * a.cstride, tstep = a.tstride;   // 4 rows: bytes; dwords per 4 rows
-            for (int i4 = 0; i4 < a.ni4; i4++) {
-                v4i b[NJB];
-#pragma unroll
-                for (int jb = 0; jb < NJB; jb++) {
-                    const uint32_t *t = trow + 4 * jb;
-                    const uint32_t w0 = t[0], w1 = t[1], w2 = t[2], w3 = t[3], w4 = t[4];
-                    b[jb] = v4i{int(__builtin_amdgcn_alignbyte(w1, w0, bsh)), int(__builtin_amdgcn_alignbyte(w2, w1, bsh)),
-                                int(__builtin_amdgcn_alignbyte(w3, w2, bsh)), int(__builtin_amdgcn_alignbyte(w4, w3, bsh))};
-                }
-#pragma unroll
-                for (int cb = 0; cb < NVT + NJB - 1; cb++) {
-                    const v4i av = *reinterpret_cast<const v4i *>(crow + 16 * cb);
-#pragma unroll
-                    for (int jb = 0; jb < NJB; jb++) {
-                        const int v1 = cb - jb;
-                        if (v1 >= 0 && v1 < NVT) acc[v1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b[jb], acc[v1], 0, 0, 0);
-                    }
-                }
-                crow += cstep;
-                trow += tstep;
-            }
-        }
-
-        // epilogue 1: screen the band in FP32, then the exact FP64 maximum of the wave
-        const int32_t *box = a.fg.box + f * g.boxSize + int64_t(sy) * g.bw + sx;
-        const int32_t *box2 = a.fg.box2 + f * g.boxSize + int64_t(sy) * g.bw + sx;
-        float fv[NVT][4];
+        bandMfma<NVT, NJB>(acc, chip, tmpl, wave, lane, a.ni4, a.tstride, a.cstride);
+        // epilogue 1: screen the band in FP32, then the exact FP64 maximum of the wave among the points
+        // within the screen's error of its maximum (a second pass recomputes the screen)
+        const int64_t boff = f * g.boxSize + int64_t(sy) * g.bw + sx;
+        const int32_t *box = a.fg.box + boff;
+        const int32_t *box2 = a.fg.box2 + boff;
+        const float *rsig = a.fg.rsig + boff;
         float fbest = -3.0f;
         int ibest = 0x7fffffff;
 #pragma unroll
@@ -193,11 +228,9 @@ __global__ void __launch_bounds__(64 * NVT) int8Correlate(Int8Args a)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int u = 16 * wave + 4 * g4 + r, v = 16 * v1 + m16;
-                fv[v1][r] = -3.0f;
                 if (u < M && v < M) {
                     const int64_t o = int64_t(u) * g.bw + v;
-                    fv[v1][r] = screenNcc(acc[v1][r], box[o], box2[o], mom);
-                    bestOf(fbest, ibest, fv[v1][r], u * M + v);
+                    bestOf(fbest, ibest, screenNcc(acc[v1][r], box[o], rsig[o], mom), u * M + v);
                 }
             }
         waveBest(fbest, ibest);
@@ -207,12 +240,15 @@ __global__ void __launch_bounds__(64 * NVT) int8Correlate(Int8Args a)
 #pragma unroll
         for (int v1 = 0; v1 < NVT; v1++)
 #pragma unroll
-            for (int r = 0; r < 4; r++)
-                if (fv[v1][r] >= thr) {
-                    const int u = 16 * wave + 4 * g4 + r, v = 16 * v1 + m16;
+            for (int r = 0; r < 4; r++) {
+                const int u = 16 * wave + 4 * g4 + r, v = 16 * v1 + m16;
+                if (u < M && v < M) {
                     const int64_t o = int64_t(u) * g.bw + v;
-                    bestOf(dbest, dibest, exactNcc(acc[v1][r], box[o], box2[o], mom), u * M + v);
+                    const int fs = box[o];
+                    if (screenNcc(acc[v1][r], fs, rsig[o], mom) >= thr)
+                        bestOf(dbest, dibest, exactNcc(acc[v1][r], fs, box2[o], mom), u * M + v);
                 }
+            }
         waveBest(dbest, dibest);
         if (lane == 0) {
             wbest[wave].value = dbest;
@@ -271,39 +307,241 @@ __global__ void __launch_bounds__(64 * NVT) int8Correlate(Int8Args a)
             for (int i = 0; i < 3; i++)
                 for (int j = 0; j < 3; j++) fs[i][j] = inside ? f9[i * 3 + j] : 0.0;
             fs[1][1] = best;
-            writePeakF64(fs, inside, ub, vb, g.R, gy, gx, total, cnt, double(mom.n), q, a.fg.offset, a.fg.peak,
+            writePeakF64(fs, inside, ub, vb, g.R, gy, gx, total, cnt, mom.nd, q, a.fg.offset, a.fg.peak,
                          a.fg.snr, a.fg.cov);
         }
     }
 }
 
-template <int NVT, int NJB>
-void launchOne(const Int8Args &a, int blocks, size_t shmem, hipStream_t stream)
+// ---- the pipelined kernel: producer and helper wavefronts
+//
+// Waves 0 .. P-1 (P = NVT bands) only run MFMAs; waves P .. are helpers. Frames are pipelined over two
+// chip buffers and two raw-surface buffers in LDS; iteration j has two phases, each ended by a block
+// barrier:
+//   phase A  producers: the MFMA bands of frame j from chip[j % 2], kept in registers
+//            helpers:   stage the chip of frame j + 1 into chip[(j + 1) % 2]; screen frame j - 1 from
+//                       surf[(j - 1) % 2] (each helper its rows: FP32 screen, exact FP64 best of its
+//                       candidates into part[(j - 1) % 2][helper]); helper 0 also finishes frame j - 2
+//                       from part[j % 2] and surf[j % 2] (peak, SNR, 3 x 3 FP64 fit, covariance)
+//   phase B  producers: write frame j's raw surface into surf[j % 2]
+// Every buffer a phase reads was written before the previous barrier and is written again only after
+// the next one, so one frame's MFMAs overlap the staging, screening and peak fitting of its neighbours.
+
+__device__ inline void helperScreen(const Int8Args &a, const Moments &mom, const int *S, const FrameWindow &fw, int f,
+                                    int h, int nh, int lane, WaveBest *out)
 {
-    auto kern = int8Correlate<NVT, NJB>;
+    const Geometry &g = a.g;
+    const int M = g.M;
+    const int64_t boff = f * g.boxSize + int64_t(fw.sy) * g.bw + fw.sx;
+    const int32_t *box = a.fg.box + boff;
+    const int32_t *box2 = a.fg.box2 + boff;
+    const float *rsig = a.fg.rsig + boff;
+    // pass 1: the two largest screened values of each lane, with the index of the largest
+    float f1 = -3.0f, f2 = -3.0f;
+    int i1 = 0x7fffffff;
+    for (int u = h; u < M; u += nh)
+        for (int v = lane; v < M; v += kWave) {
+            const int o = u * g.bw + v;
+            const float x = screenNcc(S[u * M + v], box[o], rsig[o], mom);
+            if (x > f1) {
+                f2 = f1;
+                f1 = x;
+                i1 = u * M + v;
+            }
+            else if (x > f2)
+                f2 = x;
+        }
+    float fmax = f1;
+    int imax = i1;
+    waveBest(fmax, imax);
+    const float thr = fmax - 1e-5f * fabsf(fmax) - 1e-30f;
+    double dbest = -3.0;
+    int dibest = 0x7fffffff;
+    if (__any(f2 >= thr)) {
+        // a lane holds two candidates: every point within the screen's error of the maximum
+        for (int u = h; u < M; u += nh)
+            for (int v = lane; v < M; v += kWave) {
+                const int o = u * g.bw + v;
+                const int raw = S[u * M + v], fs = box[o];
+                if (screenNcc(raw, fs, rsig[o], mom) >= thr)
+                    bestOf(dbest, dibest, exactNcc(raw, fs, box2[o], mom), u * M + v);
+            }
+    }
+    else if (f1 >= thr) {
+        const int u = i1 / M, v = i1 - u * M, o = u * g.bw + v;
+        dbest = exactNcc(S[i1], box[o], box2[o], mom);
+        dibest = i1;
+    }
+    waveBest(dbest, dibest);
+    if (lane == 0) {
+        out->value = dbest;
+        out->index = dibest;
+    }
+}
+
+__device__ inline void helperFinish(const Int8Args &a, const Moments &mom, const int *S, const FrameWindow &fw, int f,
+                                    const WaveBest *part, int nh, int lane)
+{
+    const Geometry &g = a.g;
+    const int M = g.M;
+    if (!fw.valid) {
+        if (lane == 0) invalidateOut(fw.q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
+        return;
+    }
+    double best = -3.0;
+    int ib = 0x7fffffff;
+    for (int t = 0; t < nh; t++) bestOf(best, ib, part[t].value, part[t].index);
+    if (!(best > 0)) {
+        if (lane == 0) invalidateOut(fw.q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
+        return;
+    }
+    const int64_t boff = f * g.boxSize + int64_t(fw.sy) * g.bw + fw.sx;
+    const int32_t *box = a.fg.box + boff;
+    const int32_t *box2 = a.fg.box2 + boff;
+    const int ub = ib / M, vb = ib - ub * M;
+    const int nn = g.stat < M ? g.stat : M;
+    const int ulo = max(ub - nn / 2, 0), uhi = min(ub - nn / 2 + nn, M);
+    const int vlo = max(vb - nn / 2, 0), vhi = min(vb - nn / 2 + nn, M);
+    const int bwid = vhi - vlo, items = (uhi - ulo) * bwid;
+    double sum = 0.0;
+    for (int it = lane; it < items; it += kWave) {
+        const int u = ulo + it / bwid, v = vlo + it % bwid, o = u * g.bw + v;
+        const double d = exactNcc(S[u * M + v], box[o], box2[o], mom);
+        sum += d * d;
+    }
+    sum = waveSumD(sum);
+    const bool inside = ub > 0 && ub < M - 1 && vb > 0 && vb < M - 1;
+    double mine = 0.0;
+    if (inside && lane < 9) {
+        const int u = ub + lane / 3 - 1, v = vb + lane % 3 - 1, o = u * g.bw + v;
+        mine = exactNcc(S[u * M + v], box[o], box2[o], mom);
+    }
+    double fs[3][3];
+    for (int i = 0; i < 9; i++) fs[i / 3][i % 3] = __shfl(mine, i, kWave);
+    fs[1][1] = best;
+    if (lane == 0)
+        writePeakF64(fs, inside, ub, vb, g.R, fw.gy, fw.gx, sum, items, mom.nd, fw.q, a.fg.offset, a.fg.peak,
+                     a.fg.snr, a.fg.cov);
+}
+
+template <int NVT, int NJB>
+__global__ void __launch_bounds__(64 * kPipedWaves) int8Piped(Int8Args a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const Geometry &g = a.g;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6, nthreads = blockDim.x;
+    const int P = a.nut, nh = nthreads / kWave - P;
+    const bool producer = wave < P;
+    const int h = wave - P, htid = tid - kWave * P, hthreads = kWave * nh;
+    const int logical = xcdRemap(blockIdx.x, gridDim.x);
+    const int k = logical / a.nfb, fblock = logical - k * a.nfb;
+    const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n), F = fEnd - fBegin;
+    const int W = g.W, M = g.M, L = g.L;
+
+    const int tbytes = 4 * a.ni4 * a.tstride, cbytes = a.crows * a.cstride, sints = (M * M + 3) & ~3;
+    int8_t *tmpl = reinterpret_cast<int8_t *>(lds);
+    int8_t *chips[2] = {tmpl + tbytes, tmpl + tbytes + cbytes};
+    int *surfs[2] = {reinterpret_cast<int *>(tmpl + tbytes + 2 * cbytes),
+                     reinterpret_cast<int *>(tmpl + tbytes + 2 * cbytes) + sints};
+    WaveBest *parts = reinterpret_cast<WaveBest *>(surfs[1] + sints);   // 2 x nh
+
+    loadTemplate(lds, tbytes + 2 * cbytes, tmpl, a.tmpl + size_t(k) * W * W, W, a.tstride, tid, nthreads);
+    const float normk = a.wd.norm[k];
+    const int y0 = a.wd.down[k / g.nx], x0 = a.wd.across[k % g.nx];
+    Moments mom;
+    mom.n = int64_t(W) * W;
+    mom.tvar = a.wd.tvar[k];
+    mom.nd = double(mom.n);
+    mom.tau = double(a.wd.tsum[k] - 128 * mom.n);
+    mom.rst = mom.tvar > 0 ? float(1.0 / sqrt(double(mom.tvar))) : 0.0f;
+    if (!producer && F > 0) {
+        const FrameWindow fw = frameWindow(a, k, fBegin, y0, x0, normk);
+        if (fw.valid)
+            loadChip(chips[0], a.fg.frames + size_t(fBegin) * g.pixels + size_t(fw.sy) * g.w + fw.sx, g.w, L,
+                     a.cstride, htid, hthreads);
+    }
+    __syncthreads();
+
+    const int g4 = lane >> 4, m16 = lane & 15;
+    for (int j = 0; j < F + 2; j++) {
+        v4i acc[NVT];
+        bool wrote = false;
+        // phase A
+        if (producer) {
+            if (j < F) {
+                const FrameWindow fw = frameWindow(a, k, fBegin + j, y0, x0, normk);
+                if (fw.valid) {
+                    bandMfma<NVT, NJB>(acc, chips[j & 1], tmpl, wave, lane, a.ni4, a.tstride, a.cstride);
+                    wrote = true;
+                }
+            }
+        }
+        else {
+            if (j + 1 < F) {
+                const int f = fBegin + j + 1;
+                const FrameWindow fw = frameWindow(a, k, f, y0, x0, normk);
+                if (fw.valid)
+                    loadChip(chips[(j + 1) & 1], a.fg.frames + size_t(f) * g.pixels + size_t(fw.sy) * g.w + fw.sx,
+                             g.w, L, a.cstride, htid, hthreads);
+            }
+            if (j >= 1 && j <= F) {
+                const int f = fBegin + j - 1;
+                const FrameWindow fw = frameWindow(a, k, f, y0, x0, normk);
+                if (fw.valid) helperScreen(a, mom, surfs[(j - 1) & 1], fw, f, h, nh, lane, parts + ((j - 1) & 1) * nh + h);
+            }
+            if (h == 0 && j >= 2) {
+                const int f = fBegin + j - 2;
+                const FrameWindow fw = frameWindow(a, k, f, y0, x0, normk);
+                helperFinish(a, mom, surfs[j & 1], fw, f, parts + (j & 1) * nh, nh, lane);
+            }
+        }
+        __syncthreads();
+        // phase B
+        if (wrote) {
+            int *S = surfs[j & 1];
+#pragma unroll
+            for (int v1 = 0; v1 < NVT; v1++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int u = 16 * wave + 4 * g4 + r, v = 16 * v1 + m16;
+                    if (u < M && v < M) S[u * M + v] = acc[v1][r];
+                }
+        }
+        __syncthreads();
+    }
+}
+
+template <int NVT, int NJB>
+void launchOne(const Int8Args &a, bool piped, int blocks, size_t shmem, hipStream_t stream)
+{
     static bool configured = false;
     if (!configured) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(int8Correlate<NVT, NJB>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(int8Piped<NVT, NJB>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipGetLastError();
         configured = true;
     }
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kWave * a.nut), shmem, stream, a);
+    if (piped)
+        hipLaunchKernelGGL((int8Piped<NVT, NJB>), dim3(blocks), dim3(kWave * kPipedWaves), shmem, stream, a);
+    else
+        hipLaunchKernelGGL((int8Correlate<NVT, NJB>), dim3(blocks), dim3(kWave * a.nut), shmem, stream, a);
 }
 
 template <int NJB>
-bool dispatchNvt(int nvt, const Int8Args &a, int blocks, size_t shmem, hipStream_t stream)
+bool dispatchNvt(int nvt, const Int8Args &a, bool piped, int blocks, size_t shmem, hipStream_t stream)
 {
     switch (nvt) {
-    case 1: launchOne<1, NJB>(a, blocks, shmem, stream); return true;
-    case 2: launchOne<2, NJB>(a, blocks, shmem, stream); return true;
-    case 3: launchOne<3, NJB>(a, blocks, shmem, stream); return true;
-    case 4: launchOne<4, NJB>(a, blocks, shmem, stream); return true;
-    case 5: launchOne<5, NJB>(a, blocks, shmem, stream); return true;
-    case 6: launchOne<6, NJB>(a, blocks, shmem, stream); return true;
-    case 7: launchOne<7, NJB>(a, blocks, shmem, stream); return true;
-    case 8: launchOne<8, NJB>(a, blocks, shmem, stream); return true;
-    case 9: launchOne<9, NJB>(a, blocks, shmem, stream); return true;
+    case 1: launchOne<1, NJB>(a, piped, blocks, shmem, stream); return true;
+    case 2: launchOne<2, NJB>(a, piped, blocks, shmem, stream); return true;
+    case 3: launchOne<3, NJB>(a, piped, blocks, shmem, stream); return true;
+    case 4: launchOne<4, NJB>(a, piped, blocks, shmem, stream); return true;
+    case 5: launchOne<5, NJB>(a, piped, blocks, shmem, stream); return true;
+    case 6: launchOne<6, NJB>(a, piped, blocks, shmem, stream); return true;
+    case 7: launchOne<7, NJB>(a, piped, blocks, shmem, stream); return true;
+    case 8: launchOne<8, NJB>(a, piped, blocks, shmem, stream); return true;
+    case 9: launchOne<9, NJB>(a, piped, blocks, shmem, stream); return true;
     default: return false;
     }
 }
@@ -328,7 +566,10 @@ Int8Layout int8Layout(const Geometry &g)
     l.nvt = (g.M + 15) / 16;
     l.nut = l.nvt;
     l.ni4 = (g.W + 3) / 4;
-    l.tstride = 16 * (l.njb + 2);
+    // template rows: at least 16 (NJB + 2) bytes, and 32 + 128 k bytes so that the four rows a
+    // fragment build reads at once (the four lane groups) fall on disjoint LDS banks
+    l.tstride = 32;
+    while (l.tstride < 16 * (l.njb + 2)) l.tstride += 128;
     int cs = 16 * (l.nvt + l.njb - 1);
     if (cs < g.L) cs = g.L;
     cs = (cs + 15) / 16;
@@ -338,10 +579,15 @@ Int8Layout int8Layout(const Geometry &g)
     l.shmem = size_t(4 * l.ni4) * l.tstride + size_t(l.crows) * l.cstride + sizeof(WaveBest) * l.nut
         + sizeof(double) * (l.nut + 9) + sizeof(int) * l.nut;
     l.supported = l.njb > 0 && l.nvt <= 9 && l.shmem <= 160 * 1024;
+    // the pipelined kernel: two chips, two raw surfaces, two sets of helper results
+    const int nh = kPipedWaves - l.nut;
+    l.pipedShmem = size_t(4 * l.ni4) * l.tstride + 2 * size_t(l.crows) * l.cstride
+        + 2 * sizeof(int) * size_t((g.M * g.M + 3) & ~3) + 2 * sizeof(WaveBest) * size_t(nh > 0 ? nh : 0);
+    l.pipedOk = l.supported && nh >= 2 && l.pipedShmem <= 160 * 1024;
     return l;
 }
 
-void launchInt8(Int8Args a, const Int8Layout &l, hipStream_t stream)
+void launchInt8(Int8Args a, const Int8Layout &l, Int8Variant variant, hipStream_t stream)
 {
     if (a.fg.n <= 0) return;
     if (!l.supported) throw std::invalid_argument("framecor: the int8 form supports windows up to 129 pixels and search ranges up to 71 pixels");
@@ -358,12 +604,14 @@ void launchInt8(Int8Args a, const Int8Layout &l, hipStream_t stream)
     a.fpb = (a.fg.n + nfb - 1) / nfb;
     a.nfb = (a.fg.n + a.fpb - 1) / a.fpb;
     const int blocks = a.g.nwin * a.nfb;
+    const bool piped = variant == Int8Variant::Piped ? l.pipedOk : variant == Int8Variant::Auto && l.pipedOk;
+    const size_t shmem = piped ? l.pipedShmem : l.shmem;
     bool ok = false;
     switch (l.njb) {
-    case 2: ok = dispatchNvt<2>(l.nvt, a, blocks, l.shmem, stream); break;
-    case 3: ok = dispatchNvt<3>(l.nvt, a, blocks, l.shmem, stream); break;
-    case 5: ok = dispatchNvt<5>(l.nvt, a, blocks, l.shmem, stream); break;
-    case 9: ok = dispatchNvt<9>(l.nvt, a, blocks, l.shmem, stream); break;
+    case 2: ok = dispatchNvt<2>(l.nvt, a, piped, blocks, shmem, stream); break;
+    case 3: ok = dispatchNvt<3>(l.nvt, a, piped, blocks, shmem, stream); break;
+    case 5: ok = dispatchNvt<5>(l.nvt, a, piped, blocks, shmem, stream); break;
+    case 9: ok = dispatchNvt<9>(l.nvt, a, piped, blocks, shmem, stream); break;
     default: break;
     }
     if (!ok) throw std::invalid_argument("framecor: no int8 kernel for this window and search range");

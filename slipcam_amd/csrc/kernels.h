@@ -9,9 +9,10 @@ namespace framecor {
 
 // the (h + 1) x (w + 1) int64 tables of n frames (n x h x w uint8), pixels and squares
 void launchSummedArea(const uint8_t *frames, int n, int h, int w, int64_t *sat, int64_t *sat2, hipStream_t stream);
-// the W x W box sums of n frames from their tables, n x (h - W + 1) x (w - W + 1) int32 each
+// the W x W box sums of n frames from their tables, n x (h - W + 1) x (w - W + 1) int32 each, and
+// (rsig not null) 1 / sqrt(W^2 box2 - box^2) as float (0 where that is not positive)
 void launchBoxSums(const int64_t *sat, const int64_t *sat2, int n, int h, int w, int W, int32_t *box,
-                   int32_t *box2, hipStream_t stream);
+                   int32_t *box2, float *rsig, hipStream_t stream);
 
 // ---- the int8-MFMA form (int8_corr.hip)
 
@@ -30,12 +31,17 @@ struct Int8Args {
 
 struct Int8Layout {
     int njb, nvt, nut, ni4, tstride, cstride, crows;
-    size_t shmem;
+    size_t shmem;         // the one-frame-at-a-time kernel
     bool supported;
+    size_t pipedShmem;    // the pipelined producer/helper kernel
+    bool pipedOk;
 };
 
+// Auto: the pipelined kernel where its buffers fit in LDS, else the one-frame-at-a-time kernel
+enum class Int8Variant { Auto = 0, Simple = 1, Piped = 2 };
+
 Int8Layout int8Layout(const Geometry &g);
-void launchInt8(Int8Args a, const Int8Layout &l, hipStream_t stream);
+void launchInt8(Int8Args a, const Int8Layout &l, Int8Variant variant, hipStream_t stream);
 
 // ---- the FP32 FFT form (fft_corr.hip)
 

@@ -64,6 +64,8 @@ def _load():
     lib.framecor_ncc_backward.argtypes = [P, P, I, P, P, P, P, P, P]
     lib.framecor_ncc_refine.restype = I
     lib.framecor_ncc_refine.argtypes = [P, P, I, P, P, I, P, P, P]
+    lib.framecor_set_variant.restype = I
+    lib.framecor_set_variant.argtypes = [P, I]
     lib.framecor_set_timing.restype = I
     lib.framecor_set_timing.argtypes = [P, I]
     lib.framecor_get_timing.restype = I
@@ -199,6 +201,10 @@ class Correlator:
             raise ValueError('framecor: the reference frame must be height x width')
         _check(_lib.framecor_set_reference(self._c, _ptr(frame)))
         self._ready = True
+
+    def set_variant(self, variant):
+        """The int8 kernel: 'auto', 'simple' (one frame at a time) or 'piped' (producer/helper pipeline)"""
+        _check(_lib.framecor_set_variant(self._c, {'auto': 0, 'simple': 1, 'piped': 2}[variant]))
 
     def set_timing(self, enable=True):
         """Record the GPU time of each stage of the next correlate calls (hipEvents on their stream)"""

@@ -111,6 +111,16 @@ def test_shift_gradient_central_differences(gpu, small):
         assert np.abs(fd32 - analytic[..., axis]).max() < 5e-3
 
 
+def test_refine_without_steps_is_the_forward(gpu, small):
+    """Zero Gauss-Newton steps return the start shifts and the forward NCC at them"""
+    d, down, across, shifts, gross, cor = small
+    frames = torch.tensor(d['frames'], dtype=torch.float32, device='cuda')
+    s = torch.tensor(shifts, dtype=torch.float32, device='cuda')
+    out, ncc = autograd.refine(cor, frames, s, gross=gross, steps=0)
+    assert torch.equal(out, s)
+    assert torch.allclose(ncc, autograd.ncc_shift(cor, frames, s, gross=gross), rtol=0, atol=1e-6)
+
+
 def test_gauss_newton_beside_quadratic_fit(gpu):
     """Three Gauss-Newton steps from the correlator's quadratic-fit offsets, against the known shifts
     of the smooth-field workload (a reduced 1080p shape: 270 x 480, window 64, search 32, skip 32)"""

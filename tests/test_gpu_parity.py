@@ -203,3 +203,27 @@ def test_grouping_and_device_calls_agree(gpu, fault, method):
     for key in whole:
         assert np.array_equal(whole[key], np.concatenate([o[key] for o in one]), equal_nan=True), key
         assert np.array_equal(whole[key], dev[key].cpu().numpy(), equal_nan=True), key
+
+
+@pytest.mark.parametrize('workload', ['small', 'fault'])
+def test_int8_kernel_variants_agree(gpu, small, fault, workload):
+    """The pipelined producer/helper kernel and the one-frame-at-a-time kernel: the same outputs (the
+    SNR sums are reduced in another order: equal to 1e-12)"""
+    if workload == 'small':
+        d, down, across, _, gross = small
+        h, w, W, R, frames, g = 256, 320, d['window'], d['search'], d['frames'], gross
+    else:
+        d, down, across, _ = fault
+        h, w, W, R, frames, g = 720, 1280, 64, 48, d['frames'], None
+    out = {}
+    for variant in ('simple', 'piped'):
+        cor = framecor.Correlator(h, w, down, across, window=W, search=R, method='int8')
+        cor.set_variant(variant)
+        cor.set_reference(d['ref'])
+        out[variant] = cor.correlate(frames, gross=g)
+    a, b = out['simple'], out['piped']
+    for key in ('offsets', 'peak', 'cov'):
+        assert np.array_equal(a[key], b[key], equal_nan=True), key
+    ok = a['snr'] > 0
+    assert np.array_equal(ok, b['snr'] > 0)
+    assert np.abs(a['snr'][ok] / b['snr'][ok] - 1).max() < 1e-12
