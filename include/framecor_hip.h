@@ -115,9 +115,9 @@ int framecor_ncc_backward(framecor_correlator *c, const float *frames, int n, co
 int framecor_ncc_refine(framecor_correlator *c, const float *frames, int n, const int32_t *gross,
                         const float *shifts_in, int steps, float *shifts_out, float *ncc, void *stream);
 
-/* the kernel of the int8 method (no counterpart in the reference; a tuning knob): 0 automatic (the
- * pipelined producer/helper kernel where its LDS buffers fit, else the one-frame-at-a-time kernel),
- * 1 one frame at a time, 2 pipelined (falls back to 1 where it does not fit); the results are the same */
+/* the kernel of the int8 method (no counterpart in the reference; a tuning knob): 0 automatic (one
+ * frame at a time), 1 one frame at a time, 2 the pipelined producer/helper kernel (falls back to 1
+ * where its LDS buffers do not fit); the results are the same */
 int framecor_set_variant(framecor_correlator *c, int variant);
 
 /* per-stage GPU timing (no counterpart in the reference; for benchmarks): with enable 1, the

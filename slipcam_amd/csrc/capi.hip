@@ -126,7 +126,7 @@ struct framecor_correlator {
     // per window, written by set_reference
     DeviceBuffer dDown, dAcross, dNorm, dTsum, dTvar, dTmpl, dSpectra, dTwiddle;
     // per group of frames
-    DeviceBuffer dSat, dSat2, dBox, dBox2, dRsig;
+    DeviceBuffer dSat, dSat2, dBox, dBox2, dScreen;
     int groupCap = 0;
     // host-call staging
     DeviceBuffer dFrames, dGross, dOffset, dPeak, dSnr, dCov;
@@ -157,7 +157,7 @@ struct framecor_correlator {
 
     size_t bytesPerFrame() const
     {
-        return size_t(g.satSize) * 16 + size_t(g.boxSize) * 12;
+        return size_t(g.satSize) * 16 + size_t(g.boxSize) * 16;
     }
 
     void ensureGroup(int n)
@@ -168,7 +168,7 @@ struct framecor_correlator {
         dSat2.ensure(size_t(want) * g.satSize * sizeof(int64_t));
         dBox.ensure(size_t(want) * g.boxSize * sizeof(int32_t));
         dBox2.ensure(size_t(want) * g.boxSize * sizeof(int32_t));
-        if (method == FRAMECOR_METHOD_INT8) dRsig.ensure(size_t(want) * g.boxSize * sizeof(float));
+        if (method == FRAMECOR_METHOD_INT8) dScreen.ensure(size_t(want) * g.boxSize * sizeof(int2));
         groupCap = want;
     }
 
@@ -191,7 +191,7 @@ struct framecor_correlator {
         launchSummedArea(frames, n, g.h, g.w, dSat.as<int64_t>(), dSat2.as<int64_t>(), s);
         mark(s);
         launchBoxSums(dSat.as<int64_t>(), dSat2.as<int64_t>(), n, g.h, g.w, g.W, dBox.as<int32_t>(),
-                      dBox2.as<int32_t>(), method == FRAMECOR_METHOD_INT8 ? dRsig.as<float>() : nullptr, s);
+                      dBox2.as<int32_t>(), method == FRAMECOR_METHOD_INT8 ? dScreen.as<int2>() : nullptr, s);
         mark(s);
         FrameGroup fg;
         fg.frames = frames;
@@ -199,7 +199,7 @@ struct framecor_correlator {
         fg.sat = dSat.as<int64_t>();
         fg.box = dBox.as<int32_t>();
         fg.box2 = dBox2.as<int32_t>();
-        fg.rsig = method == FRAMECOR_METHOD_INT8 ? dRsig.as<float>() : nullptr;
+        fg.screen = method == FRAMECOR_METHOD_INT8 ? dScreen.as<int2>() : nullptr;
         fg.n = n;
         fg.offset = offset;
         fg.peak = peak;

@@ -39,7 +39,7 @@ struct FrameGroup {
     const int *gross;           // n x nwin x 2, or null
     const int64_t *sat;         // n x (h + 1) x (w + 1), the summed-area table of the pixels
     const int32_t *box, *box2;  // n x bh x bw: W x W box sums of the pixels and of their squares
-    const float *rsig;          // n x bh x bw: 1 / sqrt(W^2 box2 - box^2), 0 for a flat box (int8 form)
+    const int2 *screen;         // n x bh x bw: {box, 1 / sqrt(W^2 box2 - box^2) float bits} (int8 form)
     int n;
     float *offset, *peak, *snr, *cov;
 };

@@ -169,188 +169,56 @@ __device__ inline FrameWindow frameWindow(const Int8Args &a, int k, int f, int y
     return fw;
 }
 
-template <int NVT, int NJB>
-__global__ void __launch_bounds__(64 * NVT) int8Correlate(Int8Args a)
-{
-    extern __shared__ __align__(16) unsigned char lds[];
-    const Geometry &g = a.g;
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6, nthreads = blockDim.x;
-    const int logical = xcdRemap(blockIdx.x, gridDim.x);
-    const int k = logical / a.nfb, fblock = logical - k * a.nfb;
-    const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n);
-    const int W = g.W, M = g.M, L = g.L;
+// floor(x / d) for 0 <= x < 2^20, 1 <= d < 2^12 by a multiply-high with m = ceil(2^32 / d) (d > 1)
+__device__ inline int quot(int x, unsigned m, int d) { return d == 1 ? x : int(__umulhi(unsigned(x), m)); }
+__device__ inline unsigned quotMagic(int d) { return unsigned((0x100000000ull + d - 1) / unsigned(d)); }
 
-    int8_t *tmpl = reinterpret_cast<int8_t *>(lds);
-    int8_t *chip = tmpl + 4 * a.ni4 * a.tstride;
-    WaveBest *wbest = reinterpret_cast<WaveBest *>(chip + a.crows * a.cstride);
-    double *psum = reinterpret_cast<double *>(wbest + a.nut);
-    double *f9 = psum + a.nut;
-    int *pcnt = reinterpret_cast<int *>(f9 + 9);
+constexpr int kBatch = 8;   // surface points per lane whose loads are in flight together
 
-    loadTemplate(lds, 4 * a.ni4 * a.tstride + a.crows * a.cstride, tmpl, a.tmpl + size_t(k) * W * W, W, a.tstride,
-                 tid, nthreads);
-    __syncthreads();
-
-    const float normk = a.wd.norm[k];
-    const int y0 = a.wd.down[k / g.nx], x0 = a.wd.across[k % g.nx];
-    Moments mom;
-    mom.n = int64_t(W) * W;
-    mom.tvar = a.wd.tvar[k];
-    mom.nd = double(mom.n);
-    mom.tau = double(a.wd.tsum[k] - 128 * mom.n);
-    mom.rst = mom.tvar > 0 ? float(1.0 / sqrt(double(mom.tvar))) : 0.0f;
-    const int g4 = lane >> 4, m16 = lane & 15;
-
-    for (int f = fBegin; f < fEnd; f++) {
-        const FrameWindow fw = frameWindow(a, k, f, y0, x0, normk);
-        const int64_t q = fw.q;
-        const int gy = fw.gy, gx = fw.gx, sy = fw.sy, sx = fw.sx;
-        if (!fw.valid) {
-            if (tid == 0) invalidateOut(q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
-            continue;   // uniform over the block
-        }
-        // the chip (the previous frame's reads ended at B3)
-        loadChip(chip, a.fg.frames + size_t(f) * g.pixels + size_t(sy) * g.w + sx, g.w, L, a.cstride, tid, nthreads);
-        __syncthreads();   // B1
-
-        v4i acc[NVT];
-        bandMfma<NVT, NJB>(acc, chip, tmpl, wave, lane, a.ni4, a.tstride, a.cstride);
-        // epilogue 1: screen the band in FP32, then the exact FP64 maximum of the wave among the points
-        // within the screen's error of its maximum (a second pass recomputes the screen)
-        const int64_t boff = f * g.boxSize + int64_t(sy) * g.bw + sx;
-        const int32_t *box = a.fg.box + boff;
-        const int32_t *box2 = a.fg.box2 + boff;
-        const float *rsig = a.fg.rsig + boff;
-        float fbest = -3.0f;
-        int ibest = 0x7fffffff;
-#pragma unroll
-        for (int v1 = 0; v1 < NVT; v1++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int u = 16 * wave + 4 * g4 + r, v = 16 * v1 + m16;
-                if (u < M && v < M) {
-                    const int64_t o = int64_t(u) * g.bw + v;
-                    bestOf(fbest, ibest, screenNcc(acc[v1][r], box[o], rsig[o], mom), u * M + v);
-                }
-            }
-        waveBest(fbest, ibest);
-        const float thr = fbest - 1e-5f * fabsf(fbest) - 1e-30f;
-        double dbest = -3.0;
-        int dibest = 0x7fffffff;
-#pragma unroll
-        for (int v1 = 0; v1 < NVT; v1++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int u = 16 * wave + 4 * g4 + r, v = 16 * v1 + m16;
-                if (u < M && v < M) {
-                    const int64_t o = int64_t(u) * g.bw + v;
-                    const int fs = box[o];
-                    if (screenNcc(acc[v1][r], fs, rsig[o], mom) >= thr)
-                        bestOf(dbest, dibest, exactNcc(acc[v1][r], fs, box2[o], mom), u * M + v);
-                }
-            }
-        waveBest(dbest, dibest);
-        if (lane == 0) {
-            wbest[wave].value = dbest;
-            wbest[wave].index = dibest;
-        }
-        __syncthreads();   // B2
-
-        // epilogue 2: every wave finds the block's peak; the bands that meet the stat box add their
-        // exact squared surface, and the 3 x 3 around the peak
-        double best = -3.0;
-        int ib = 0x7fffffff;
-        for (int t = 0; t < a.nut; t++) bestOf(best, ib, wbest[t].value, wbest[t].index);
-        if (!(best > 0)) {   // uniform: every thread read the same values
-            if (tid == 0) invalidateOut(q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
-            continue;
-        }
-        const int ub = ib / M, vb = ib - ub * M;
-        const int nn = g.stat < M ? g.stat : M;
-        const int ulo = max(ub - nn / 2, 0), uhi = min(ub - nn / 2 + nn, M);
-        const int vlo = max(vb - nn / 2, 0), vhi = min(vb - nn / 2 + nn, M);
-        double sum = 0.0;
-        int count = 0;
-        if (16 * wave + 15 >= ulo && 16 * wave < uhi) {
-#pragma unroll
-            for (int v1 = 0; v1 < NVT; v1++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int u = 16 * wave + 4 * g4 + r, v = 16 * v1 + m16;
-                    if (u >= ulo && u < uhi && v >= vlo && v < vhi) {
-                        const int64_t o = int64_t(u) * g.bw + v;
-                        const double d = exactNcc(acc[v1][r], box[o], box2[o], mom);
-                        sum += d * d;
-                        count++;
-                        if (u >= ub - 1 && u <= ub + 1 && v >= vb - 1 && v <= vb + 1)
-                            f9[(u - ub + 1) * 3 + (v - vb + 1)] = d;
-                    }
-                }
-            sum = waveSumD(sum);
-            count = waveSumI(count);
-        }
-        if (lane == 0) {
-            psum[wave] = sum;
-            pcnt[wave] = count;
-        }
-        __syncthreads();   // B3
-
-        if (tid == 0) {
-            double total = 0.0;
-            int cnt = 0;
-            for (int t = 0; t < a.nut; t++) {
-                total += psum[t];
-                cnt += pcnt[t];
-            }
-            const bool inside = ub > 0 && ub < M - 1 && vb > 0 && vb < M - 1;
-            double fs[3][3];
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) fs[i][j] = inside ? f9[i * 3 + j] : 0.0;
-            fs[1][1] = best;
-            writePeakF64(fs, inside, ub, vb, g.R, gy, gx, total, cnt, mom.nd, q, a.fg.offset, a.fg.peak,
-                         a.fg.snr, a.fg.cov);
-        }
-    }
-}
-
-// ---- the pipelined kernel: producer and helper wavefronts
-//
-// Waves 0 .. P-1 (P = NVT bands) only run MFMAs; waves P .. are helpers. Frames are pipelined over two
-// chip buffers and two raw-surface buffers in LDS; iteration j has two phases, each ended by a block
-// barrier:
-//   phase A  producers: the MFMA bands of frame j from chip[j % 2], kept in registers
-//            helpers:   stage the chip of frame j + 1 into chip[(j + 1) % 2]; screen frame j - 1 from
-//                       surf[(j - 1) % 2] (each helper its rows: FP32 screen, exact FP64 best of its
-//                       candidates into part[(j - 1) % 2][helper]); helper 0 also finishes frame j - 2
-//                       from part[j % 2] and surf[j % 2] (peak, SNR, 3 x 3 FP64 fit, covariance)
-//   phase B  producers: write frame j's raw surface into surf[j % 2]
-// Every buffer a phase reads was written before the previous barrier and is written again only after
-// the next one, so one frame's MFMAs overlap the staging, screening and peak fitting of its neighbours.
-
+// helper wave h of nh: the rows h, h + nh, ... of the raw surface S of frame f: the FP32 screen, then
+// the exact FP64 best among the points within the screen's error of the wave's maximum, into *out
 __device__ inline void helperScreen(const Int8Args &a, const Moments &mom, const int *S, const FrameWindow &fw, int f,
                                     int h, int nh, int lane, WaveBest *out)
 {
     const Geometry &g = a.g;
     const int M = g.M;
     const int64_t boff = f * g.boxSize + int64_t(fw.sy) * g.bw + fw.sx;
-    const int32_t *box = a.fg.box + boff;
+    const int2 *scr = a.fg.screen + boff;
     const int32_t *box2 = a.fg.box2 + boff;
-    const float *rsig = a.fg.rsig + boff;
+    const int nrows = (M - h + nh - 1) / nh, total = nrows * M;
+    const unsigned mM = quotMagic(M);
     // pass 1: the two largest screened values of each lane, with the index of the largest
     float f1 = -3.0f, f2 = -3.0f;
     int i1 = 0x7fffffff;
-    for (int u = h; u < M; u += nh)
-        for (int v = lane; v < M; v += kWave) {
-            const int o = u * g.bw + v;
-            const float x = screenNcc(S[u * M + v], box[o], rsig[o], mom);
-            if (x > f1) {
-                f2 = f1;
-                f1 = x;
-                i1 = u * M + v;
+    for (int e0 = lane; e0 < total; e0 += kBatch * kWave) {
+        int raw[kBatch], fs[kBatch], at[kBatch];
+        float rs[kBatch];
+#pragma unroll
+        for (int t = 0; t < kBatch; t++) {
+            const int e = e0 + t * kWave;
+            at[t] = -1;
+            if (e < total) {
+                const int r = quot(e, mM, M), v = e - r * M, u = h + r * nh;
+                at[t] = u * M + v;
+                raw[t] = S[at[t]];
+                const int2 p = scr[u * g.bw + v];
+                fs[t] = p.x;
+                rs[t] = __int_as_float(p.y);
             }
-            else if (x > f2)
-                f2 = x;
         }
+#pragma unroll
+        for (int t = 0; t < kBatch; t++)
+            if (at[t] >= 0) {
+                const float x = screenNcc(raw[t], fs[t], rs[t], mom);
+                if (x > f1 || (x == f1 && at[t] < i1)) {
+                    f2 = f1;
+                    f1 = x;
+                    i1 = at[t];
+                }
+                else if (x > f2)
+                    f2 = x;
+            }
+    }
     float fmax = f1;
     int imax = i1;
     waveBest(fmax, imax);
@@ -359,17 +227,17 @@ __device__ inline void helperScreen(const Int8Args &a, const Moments &mom, const
     int dibest = 0x7fffffff;
     if (__any(f2 >= thr)) {
         // a lane holds two candidates: every point within the screen's error of the maximum
-        for (int u = h; u < M; u += nh)
-            for (int v = lane; v < M; v += kWave) {
-                const int o = u * g.bw + v;
-                const int raw = S[u * M + v], fs = box[o];
-                if (screenNcc(raw, fs, rsig[o], mom) >= thr)
-                    bestOf(dbest, dibest, exactNcc(raw, fs, box2[o], mom), u * M + v);
-            }
+        for (int e = lane; e < total; e += kWave) {
+            const int r = quot(e, mM, M), v = e - r * M, u = h + r * nh;
+            const int2 p = scr[u * g.bw + v];
+            const int raw = S[u * M + v];
+            if (screenNcc(raw, p.x, __int_as_float(p.y), mom) >= thr)
+                bestOf(dbest, dibest, exactNcc(raw, p.x, box2[u * g.bw + v], mom), u * M + v);
+        }
     }
     else if (f1 >= thr) {
         const int u = i1 / M, v = i1 - u * M, o = u * g.bw + v;
-        dbest = exactNcc(S[i1], box[o], box2[o], mom);
+        dbest = exactNcc(S[i1], scr[o].x, box2[o], mom);
         dibest = i1;
     }
     waveBest(dbest, dibest);
@@ -379,6 +247,8 @@ __device__ inline void helperScreen(const Int8Args &a, const Moments &mom, const
     }
 }
 
+// one wave: the peak of frame f from the helpers' results, then the SNR over the stat box and the 3 x 3
+// FP64 fit and covariance on exact values (cxx/surface.h:71-146 formulas)
 __device__ inline void helperFinish(const Int8Args &a, const Moments &mom, const int *S, const FrameWindow &fw, int f,
                                     const WaveBest *part, int nh, int lane)
 {
@@ -396,33 +266,126 @@ __device__ inline void helperFinish(const Int8Args &a, const Moments &mom, const
         return;
     }
     const int64_t boff = f * g.boxSize + int64_t(fw.sy) * g.bw + fw.sx;
-    const int32_t *box = a.fg.box + boff;
+    const int2 *scr = a.fg.screen + boff;
     const int32_t *box2 = a.fg.box2 + boff;
     const int ub = ib / M, vb = ib - ub * M;
     const int nn = g.stat < M ? g.stat : M;
     const int ulo = max(ub - nn / 2, 0), uhi = min(ub - nn / 2 + nn, M);
     const int vlo = max(vb - nn / 2, 0), vhi = min(vb - nn / 2 + nn, M);
     const int bwid = vhi - vlo, items = (uhi - ulo) * bwid;
-    double sum = 0.0;
-    for (int it = lane; it < items; it += kWave) {
-        const int u = ulo + it / bwid, v = vlo + it % bwid, o = u * g.bw + v;
-        const double d = exactNcc(S[u * M + v], box[o], box2[o], mom);
-        sum += d * d;
-    }
-    sum = waveSumD(sum);
     const bool inside = ub > 0 && ub < M - 1 && vb > 0 && vb < M - 1;
+    double sum = 0.0;
+    // the stat box, kBatch points per lane in flight; lanes 0..8 also take the 3 x 3 neighbours
     double mine = 0.0;
+    for (int it0 = lane; it0 < items; it0 += kBatch * kWave) {
+        int raw[kBatch], fs[kBatch], f2v[kBatch];
+        bool ok[kBatch];
+#pragma unroll
+        for (int t = 0; t < kBatch; t++) {
+            const int it = it0 + t * kWave;
+            ok[t] = it < items;
+            if (ok[t]) {
+                const int u = ulo + it / bwid, v = vlo + it % bwid, o = u * g.bw + v;
+                raw[t] = S[u * M + v];
+                fs[t] = scr[o].x;
+                f2v[t] = box2[o];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kBatch; t++)
+            if (ok[t]) {
+                const double d = exactNcc(raw[t], fs[t], f2v[t], mom);
+                sum += d * d;
+            }
+    }
     if (inside && lane < 9) {
         const int u = ub + lane / 3 - 1, v = vb + lane % 3 - 1, o = u * g.bw + v;
-        mine = exactNcc(S[u * M + v], box[o], box2[o], mom);
+        mine = exactNcc(S[u * M + v], scr[o].x, box2[o], mom);
     }
-    double fs[3][3];
-    for (int i = 0; i < 9; i++) fs[i / 3][i % 3] = __shfl(mine, i, kWave);
-    fs[1][1] = best;
+    sum = waveSumD(sum);
+    double fsq[3][3];
+    for (int i = 0; i < 9; i++) fsq[i / 3][i % 3] = __shfl(mine, i, kWave);
+    fsq[1][1] = best;
     if (lane == 0)
-        writePeakF64(fs, inside, ub, vb, g.R, fw.gy, fw.gx, sum, items, mom.nd, fw.q, a.fg.offset, a.fg.peak,
+        writePeakF64(fsq, inside, ub, vb, g.R, fw.gy, fw.gx, sum, items, mom.nd, fw.q, a.fg.offset, a.fg.peak,
                      a.fg.snr, a.fg.cov);
 }
+
+template <int NVT, int NJB>
+__global__ void __launch_bounds__(64 * NVT) int8Correlate(Int8Args a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const Geometry &g = a.g;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6, nthreads = blockDim.x;
+    const int nw = nthreads / kWave;
+    const int logical = xcdRemap(blockIdx.x, gridDim.x);
+    const int k = logical / a.nfb, fblock = logical - k * a.nfb;
+    const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n);
+    const int W = g.W, M = g.M, L = g.L;
+
+    // the raw surface after the chip, or (a.alias) over it: then the MFMAs of a frame must be over before
+    // the surface is written, and the frame finished before the next chip is staged
+    const int tbytes = 4 * a.ni4 * a.tstride, cbytes = a.crows * a.cstride, sbytes = 4 * ((M * M + 3) & ~3);
+    int8_t *tmpl = reinterpret_cast<int8_t *>(lds);
+    int8_t *chip = tmpl + tbytes;
+    int *surf = reinterpret_cast<int *>(a.alias ? chip : chip + cbytes);
+    WaveBest *parts = reinterpret_cast<WaveBest *>(chip + (a.alias ? max(cbytes, sbytes) : cbytes + sbytes));
+
+    loadTemplate(lds, tbytes + cbytes, tmpl, a.tmpl + size_t(k) * W * W, W, a.tstride, tid, nthreads);
+    __syncthreads();
+    const float normk = a.wd.norm[k];
+    const int y0 = a.wd.down[k / g.nx], x0 = a.wd.across[k % g.nx];
+    Moments mom;
+    mom.n = int64_t(W) * W;
+    mom.tvar = a.wd.tvar[k];
+    mom.nd = double(mom.n);
+    mom.tau = double(a.wd.tsum[k] - 128 * mom.n);
+    mom.rst = mom.tvar > 0 ? float(1.0 / sqrt(double(mom.tvar))) : 0.0f;
+    const int g4 = lane >> 4, m16 = lane & 15;
+
+    for (int f = fBegin; f < fEnd; f++) {
+        const FrameWindow fw = frameWindow(a, k, f, y0, x0, normk);
+        if (!fw.valid) {
+            if (tid == 0) invalidateOut(fw.q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
+            continue;   // uniform over the block
+        }
+        // the chip (the previous frame's MFMAs ended before its second barrier)
+        loadChip(chip, a.fg.frames + size_t(f) * g.pixels + size_t(fw.sy) * g.w + fw.sx, g.w, L, a.cstride, tid,
+                 nthreads);
+        __syncthreads();   // B1 (also: wave 0 has finished reading the previous surface)
+        {
+            v4i acc[NVT];
+            bandMfma<NVT, NJB>(acc, chip, tmpl, wave, lane, a.ni4, a.tstride, a.cstride);
+            if (a.alias) __syncthreads();
+#pragma unroll
+            for (int v1 = 0; v1 < NVT; v1++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int u = 16 * wave + 4 * g4 + r, v = 16 * v1 + m16;
+                    if (u < M && v < M) surf[u * M + v] = acc[v1][r];
+                }
+        }
+        __syncthreads();   // B2
+        helperScreen(a, mom, surf, fw, f, wave, nw, lane, parts + wave);
+        __syncthreads();   // B3
+        if (wave == 0) helperFinish(a, mom, surf, fw, f, parts, nw, lane);
+        if (a.alias) __syncthreads();
+    }
+}
+
+// ---- the pipelined kernel: producer and helper wavefronts
+//
+// Waves 0 .. P-1 (P = NVT bands) only run MFMAs; waves P .. are helpers. Frames are pipelined over two
+// chip buffers and two raw-surface buffers in LDS; iteration j has two phases, each ended by a block
+// barrier:
+//   phase A  producers: the MFMA bands of frame j from chip[j % 2], kept in registers
+//            helpers:   stage the chip of frame j + 1 into chip[(j + 1) % 2]; screen frame j - 1 from
+//                       surf[(j - 1) % 2] (each helper its rows: FP32 screen, exact FP64 best of its
+//                       candidates into part[(j - 1) % 2][helper]); helper 0 also finishes frame j - 2
+//                       from part[j % 2] and surf[j % 2] (peak, SNR, 3 x 3 FP64 fit, covariance)
+//   phase B  producers: write frame j's raw surface into surf[j % 2]
+// Every buffer a phase reads was written before the previous barrier and is written again only after
+// the next one, so one frame's MFMAs overlap the staging, screening and peak fitting of its neighbours.
 
 template <int NVT, int NJB>
 __global__ void __launch_bounds__(64 * kPipedWaves) int8Piped(Int8Args a)
@@ -576,8 +539,11 @@ Int8Layout int8Layout(const Geometry &g)
     if (cs % 2 == 0) cs++;   // an odd number of 16-byte units: conflict-free 16-byte row reads
     l.cstride = 16 * cs;
     l.crows = 16 * l.nut + 4 * l.ni4;
-    l.shmem = size_t(4 * l.ni4) * l.tstride + size_t(l.crows) * l.cstride + sizeof(WaveBest) * l.nut
-        + sizeof(double) * (l.nut + 9) + sizeof(int) * l.nut;
+    const size_t tb = size_t(4 * l.ni4) * l.tstride, cb = size_t(l.crows) * l.cstride;
+    const size_t sb = sizeof(int) * size_t((g.M * g.M + 3) & ~3), pb = sizeof(WaveBest) * l.nut;
+    l.shmem = tb + cb + sb + pb;
+    l.alias = l.shmem > 80 * 1024;   // two blocks per CU need at most 80 KB each; else share chip and surface
+    if (l.alias) l.shmem = tb + (cb > sb ? cb : sb) + pb;
     l.supported = l.njb > 0 && l.nvt <= 9 && l.shmem <= 160 * 1024;
     // the pipelined kernel: two chips, two raw surfaces, two sets of helper results
     const int nh = kPipedWaves - l.nut;
@@ -604,7 +570,8 @@ void launchInt8(Int8Args a, const Int8Layout &l, Int8Variant variant, hipStream_
     a.fpb = (a.fg.n + nfb - 1) / nfb;
     a.nfb = (a.fg.n + a.fpb - 1) / a.fpb;
     const int blocks = a.g.nwin * a.nfb;
-    const bool piped = variant == Int8Variant::Piped ? l.pipedOk : variant == Int8Variant::Auto && l.pipedOk;
+    const bool piped = variant == Int8Variant::Piped && l.pipedOk;
+    a.alias = l.alias ? 1 : 0;
     const size_t shmem = piped ? l.pipedShmem : l.shmem;
     bool ok = false;
     switch (l.njb) {

@@ -10,9 +10,9 @@ namespace framecor {
 // the (h + 1) x (w + 1) int64 tables of n frames (n x h x w uint8), pixels and squares
 void launchSummedArea(const uint8_t *frames, int n, int h, int w, int64_t *sat, int64_t *sat2, hipStream_t stream);
 // the W x W box sums of n frames from their tables, n x (h - W + 1) x (w - W + 1) int32 each, and
-// (rsig not null) 1 / sqrt(W^2 box2 - box^2) as float (0 where that is not positive)
+// (screen not null) {box, 1 / sqrt(W^2 box2 - box^2) as float bits (0 where that is not positive)}
 void launchBoxSums(const int64_t *sat, const int64_t *sat2, int n, int h, int w, int W, int32_t *box,
-                   int32_t *box2, float *rsig, hipStream_t stream);
+                   int32_t *box2, int2 *screen, hipStream_t stream);
 
 // ---- the int8-MFMA form (int8_corr.hip)
 
@@ -27,17 +27,20 @@ struct Int8Args {
     int cstride;           // LDS bytes per chip row
     int crows;             // LDS chip rows
     int fpb, nfb;          // frames per block, frame blocks
+    int alias;             // the one-frame kernel: the raw surface shares the chip's LDS
 };
 
 struct Int8Layout {
     int njb, nvt, nut, ni4, tstride, cstride, crows;
     size_t shmem;         // the one-frame-at-a-time kernel
+    bool alias;           // its raw surface over its chip (large windows)
     bool supported;
     size_t pipedShmem;    // the pipelined producer/helper kernel
     bool pipedOk;
 };
 
-// Auto: the pipelined kernel where its buffers fit in LDS, else the one-frame-at-a-time kernel
+// Auto: the one-frame-at-a-time kernel (two blocks per CU); Piped: the producer/helper pipeline where its
+// buffers fit in LDS
 enum class Int8Variant { Auto = 0, Simple = 1, Piped = 2 };
 
 Int8Layout int8Layout(const Geometry &g);

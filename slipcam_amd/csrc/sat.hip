@@ -9,10 +9,10 @@
 //               batched eight rows ahead;
 //   boxSums  -- the W x W box sums of the pixels and of their squares at every window position,
 //               (h - W + 1) x (w - W + 1) int32 per frame, from four table reads each (exact: a box of
-//               squares sums to at most W^2 * 65025 < 2^31 for W <= 181), and optionally the FP32
-//               reciprocal of the box's standard-deviation term, 1 / sqrt(W^2 sum F^2 - (sum F)^2) (0 for a
-//               flat box), which every window touching that position shares. The correlation kernels read
-//               these maps instead of eight int64 table entries per surface point.
+//               squares sums to at most W^2 * 65025 < 2^31 for W <= 181), and optionally a packed map of
+//               {sum F, 1 / sqrt(W^2 sum F^2 - (sum F)^2) as FP32 (0 for a flat box)} that every window
+//               touching that position shares (one 8-byte load per surface point in the int8 form's
+//               screen). The correlation kernels read these maps instead of eight int64 table entries.
 
 #include "common.h"
 #include "kernels.h"
@@ -105,7 +105,7 @@ satCols(int h, int w, int64_t satSize, int64_t *sat, int64_t *sat2)
 
 __global__ void __launch_bounds__(256)
 boxSums(const int64_t *sat, const int64_t *sat2, int h, int w, int W, int64_t satSize, int64_t boxSize,
-        int32_t *box, int32_t *box2, float *rsig)
+        int32_t *box, int32_t *box2, int2 *screen)
 {
     const int bw = w - W + 1;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,9 +120,9 @@ boxSums(const int64_t *sat, const int64_t *sat2, int h, int w, int W, int64_t sa
     const int64_t o = f * boxSize + int64_t(y) * bw + x;
     box[o] = int32_t(b);
     box2[o] = int32_t(b2);
-    if (rsig) {
+    if (screen) {
         const int64_t v = int64_t(W) * W * b2 - b * b;
-        rsig[o] = v > 0 ? float(1.0 / sqrt(double(v))) : 0.0f;
+        screen[o] = make_int2(int32_t(b), __float_as_int(v > 0 ? float(1.0 / sqrt(double(v))) : 0.0f));
     }
 }
 
@@ -140,14 +140,14 @@ void launchSummedArea(const uint8_t *frames, int n, int h, int w, int64_t *sat, 
 }
 
 void launchBoxSums(const int64_t *sat, const int64_t *sat2, int n, int h, int w, int W, int32_t *box,
-                   int32_t *box2, float *rsig, hipStream_t stream)
+                   int32_t *box2, int2 *screen, hipStream_t stream)
 {
     if (n <= 0) return;
     const int64_t satSize = int64_t(h + 1) * (w + 1);
     const int bh = h - W + 1, bw = w - W + 1;
     const int64_t boxSize = int64_t(bh) * bw;
     hipLaunchKernelGGL(boxSums, dim3((bw + 255) / 256, bh, n), dim3(256), 0, stream, sat, sat2, h, w, W,
-                       satSize, boxSize, box, box2, rsig);
+                       satSize, boxSize, box, box2, screen);
     FC_HIP_CHECK(hipGetLastError());
 }
 

@@ -158,6 +158,7 @@ class Correlator:
                                     int(window), int(search), m, int(batch), int(stat_size), int(device),
                                     ctypes.byref(handle)))
         self._c = handle
+        self._destroy = _lib.framecor_destroy
         self._h, self._w = int(height), int(width)
         self._window, self._search = int(window), int(search)
         self._shape = (len(down), len(across))
@@ -166,9 +167,9 @@ class Correlator:
         self._ready = False
 
     def __del__(self):
-        c = getattr(self, '_c', None)
-        if c:
-            _lib.framecor_destroy(c)
+        c, destroy = getattr(self, '_c', None), getattr(self, '_destroy', None)
+        if c and destroy is not None:
+            destroy(c)
             self._c = None
 
     @property
