@@ -117,7 +117,9 @@ int framecor_ncc_refine(framecor_correlator *c, const float *frames, int n, cons
 
 /* the kernel of the int8 method (no counterpart in the reference; a tuning knob): 0 automatic (one
  * frame at a time), 1 one frame at a time, 2 the pipelined producer/helper kernel (falls back to 1
- * where its LDS buffers do not fit); the results are the same */
+ * where its LDS buffers do not fit), 3 one frame at a time with ceil(M / 16) full 16-row bands (where
+ * M = 2 search + 1 = 16 c + 1, variants 0 and 1 compute c x c tiles plus the edge row and the edge
+ * column in a wave each); the offsets, peaks and covariances are the same, the SNR to rounding */
 int framecor_set_variant(framecor_correlator *c, int variant);
 
 /* per-stage GPU timing (no counterpart in the reference; for benchmarks): with enable 1, the

@@ -113,6 +113,84 @@ __device__ inline void bandMfma(v4i (&acc)[NVT], const int8_t *chip, const int8_
     }
 }
 
+// The edge row and column of a surface of M = 16 c + 1 points (every search range R = 8 c):
+// the c x c core tiles come from c bands of c tiles; the row u = 16 c and the column v = 16 c are
+// computed here instead of as a 17th band and a 17th tile column (16 rows or columns computed to keep
+// one). Every LDS read is unconditional (lanes whose outputs are discarded read clamped addresses), so
+// the loads of an iteration are issued together ahead of its MFMAs.
+//
+// The edge row: the v blocks (0..c) on the MFMA's M axis, the template Toeplitz on N as in the bands:
+//   A[m][(i, jb, kk)] = F[16 c + i][16 (m + jb) + kk],  B = the bands' fragments,  C[m][n] = r[16 c][16 m + n]
+// (rows m > c of C are discarded; their lanes read the chip at m = c).
+template <int NJB>
+__device__ inline v4i edgeRowMfma(const int8_t *chip, const int8_t *tmpl, int c, int lane, int ni4, int tstride,
+                                  int cstride)
+{
+    v4i acc = v4i{0, 0, 0, 0};   // one chain: MFMAs feeding their own accumulator issue back to back
+    const int g4 = lane >> 4, m16 = lane & 15;
+    const int bbyte = 16 - m16, bdw = bbyte >> 2, bsh = bbyte & 3;
+    const int8_t *crow = chip + (16 * c + g4) * cstride + 16 * min(m16, c);
+    const uint32_t *trow = reinterpret_cast<const uint32_t *>(tmpl + g4 * tstride) + bdw;
+    for (int i4 = 0; i4 < ni4; i4++) {
+        v4i av[NJB], b[NJB];
+#pragma unroll
+        for (int jb = 0; jb < NJB; jb++) {
+            const uint32_t *t = trow + 4 * jb;
+            const uint32_t w0 = t[0], w1 = t[1], w2 = t[2], w3 = t[3], w4 = t[4];
+            b[jb] = v4i{int(__builtin_amdgcn_alignbyte(w1, w0, bsh)), int(__builtin_amdgcn_alignbyte(w2, w1, bsh)),
+                        int(__builtin_amdgcn_alignbyte(w3, w2, bsh)), int(__builtin_amdgcn_alignbyte(w4, w3, bsh))};
+            av[jb] = *reinterpret_cast<const v4i *>(crow + 16 * jb);
+        }
+#pragma unroll
+        for (int jb = 0; jb < NJB; jb++) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[jb], b[jb], acc, 0, 0, 0);
+        crow += 4 * cstride;
+        trow += tstride;
+    }
+    return acc;
+}
+
+// The edge column: a Toeplitz expansion of the template along its columns' transpose, the u blocks
+// (0..c-1) on M and the 16 rows of a block on N:
+//   A[m][(i', jb, kk)] = F[16 m + i'][16 c + 16 jb + kk],  B[(i', jb, kk)][n] = T[i' - n][16 jb + kk],
+//   C[m][n] = sum over i' in [0, W + 15), jb in [0, ceil(W / 16))  -> r[16 m + n][16 c]
+// (i', jb) pairs are taken four at a time, one per 16-lane group. Rows m >= c of
+// C are discarded (their lanes read the chip at m = c - 1); a B fragment outside the template (i' - n
+// outside 0..W-1, or a pair past the last) reads bytes 0..15 of template row 0, which are zero.
+__device__ inline v4i edgeColMfma(const int8_t *chip, const int8_t *tmpl, int c, int W, int lane, int tstride,
+                                  int cstride)
+{
+    v4i acc = v4i{0, 0, 0, 0};
+    const int g4 = lane >> 4, m16 = lane & 15;
+    const int nJ = (W + 15) >> 4, groups = (W + 15) * nJ, steps = (groups + 3) >> 2;
+    const unsigned mJ = unsigned((0x100000000ull + nJ - 1) / unsigned(nJ));
+    const int8_t *abase = chip + 16 * min(m16, c - 1) * cstride + 16 * c;
+    int t = 0;
+    for (; t + 4 <= steps; t += 4) {
+        v4i av[4], bv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int s = 4 * (t + q) + g4;
+            const int ip = nJ == 1 ? s : int(__umulhi(unsigned(s), mJ)), jb = s - ip * nJ;
+            const int ti = ip - m16;
+            const bool inside = s < groups && ti >= 0 && ti < W;
+            av[q] = *reinterpret_cast<const v4i *>(abase + (s < groups ? ip * cstride + 16 * jb : 0));
+            bv[q] = *reinterpret_cast<const v4i *>(tmpl + (inside ? ti * tstride + 16 + 16 * jb : 0));
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[q], bv[q], acc, 0, 0, 0);
+    }
+    for (; t < steps; t++) {
+        const int s = 4 * t + g4;
+        const int ip = nJ == 1 ? s : int(__umulhi(unsigned(s), mJ)), jb = s - ip * nJ;
+        const int ti = ip - m16;
+        const bool inside = s < groups && ti >= 0 && ti < W;
+        const v4i av = *reinterpret_cast<const v4i *>(abase + (s < groups ? ip * cstride + 16 * jb : 0));
+        const v4i bv = *reinterpret_cast<const v4i *>(tmpl + (inside ? ti * tstride + 16 + 16 * jb : 0));
+        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
 // the chip of a frame (L x L at (sy, sx)) as int8 (pixels ^ 0x80 = pixels - 128) into LDS, by threads
 // t0 .. t0 + nt - 1 of the block (t: this thread's index among them)
 __device__ inline void loadChip(int8_t *chip, const uint8_t *src, int w, int L, int cstride, int t, int nt)
@@ -311,8 +389,11 @@ __device__ inline void helperFinish(const Int8Args &a, const Moments &mom, const
                      a.fg.snr, a.fg.cov);
 }
 
-template <int NVT, int NJB>
-__global__ void __launch_bounds__(64 * NVT) int8Correlate(Int8Args a)
+// EW = 0: ceil(M / 16) = NVT full bands; EW = 2 (M = 16 NVT + 1): NVT core bands plus a wave for the edge
+// row and one for the edge column (EW = 1, both in one wave, makes that wave the critical path)
+template <int NVT, int NJB, int EW>
+__global__ void __launch_bounds__(64 * (NVT + EW))
+__attribute__((amdgpu_waves_per_eu(NVT + EW <= 8 ? 4 : 1))) int8Correlate(Int8Args a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     const Geometry &g = a.g;
@@ -353,7 +434,8 @@ __global__ void __launch_bounds__(64 * NVT) int8Correlate(Int8Args a)
         loadChip(chip, a.fg.frames + size_t(f) * g.pixels + size_t(fw.sy) * g.w + fw.sx, g.w, L, a.cstride, tid,
                  nthreads);
         __syncthreads();   // B1 (also: wave 0 has finished reading the previous surface)
-        {
+        if (EW == 0 || wave < NVT) {
+            // a band: rows 16 wave .. + 15, NVT tiles (with an edge: the 16 c x 16 c core)
             v4i acc[NVT];
             bandMfma<NVT, NJB>(acc, chip, tmpl, wave, lane, a.ni4, a.tstride, a.cstride);
             if (a.alias) __syncthreads();
@@ -364,6 +446,20 @@ __global__ void __launch_bounds__(64 * NVT) int8Correlate(Int8Args a)
                     const int u = 16 * wave + 4 * g4 + r, v = 16 * v1 + m16;
                     if (u < M && v < M) surf[u * M + v] = acc[v1][r];
                 }
+        }
+        else {
+            // the edge row u = 16 c and the edge column v = 16 c (c = NVT)
+            const bool doRow = EW == 1 || wave == NVT, doCol = EW == 1 || wave == NVT + 1;
+            v4i rs = v4i{0, 0, 0, 0}, cs = v4i{0, 0, 0, 0};
+            if (doRow) rs = edgeRowMfma<NJB>(chip, tmpl, NVT, lane, a.ni4, a.tstride, a.cstride);
+            if (doCol) cs = edgeColMfma(chip, tmpl, NVT, W, lane, a.tstride, a.cstride);
+            if (a.alias) __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int blk = 4 * g4 + r, v = 16 * blk + m16;
+                if (doRow && blk <= NVT && v < M) surf[16 * NVT * M + v] = rs[r];
+                if (doCol && blk < NVT) surf[(16 * blk + m16) * M + 16 * NVT] = cs[r];
+            }
         }
         __syncthreads();   // B2
         helperScreen(a, mom, surf, fw, f, wave, nw, lane, parts + wave);
@@ -474,37 +570,57 @@ __global__ void __launch_bounds__(64 * kPipedWaves) int8Piped(Int8Args a)
     }
 }
 
-template <int NVT, int NJB>
-void launchOne(const Int8Args &a, bool piped, int blocks, size_t shmem, hipStream_t stream)
+template <int NVT, int NJB, int EW>
+void launchSimple(const Int8Args &a, int blocks, size_t shmem, hipStream_t stream)
 {
     static bool configured = false;
     if (!configured) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(int8Correlate<NVT, NJB>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(int8Correlate<NVT, NJB, EW>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipGetLastError();
+        configured = true;
+    }
+    hipLaunchKernelGGL((int8Correlate<NVT, NJB, EW>), dim3(blocks), dim3(kWave * (NVT + EW)), shmem, stream, a);
+}
+
+template <int NVT, int NJB>
+void launchPiped(const Int8Args &a, int blocks, size_t shmem, hipStream_t stream)
+{
+    static bool configured = false;
+    if (!configured) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(int8Piped<NVT, NJB>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipGetLastError();
         configured = true;
     }
-    if (piped)
-        hipLaunchKernelGGL((int8Piped<NVT, NJB>), dim3(blocks), dim3(kWave * kPipedWaves), shmem, stream, a);
-    else
-        hipLaunchKernelGGL((int8Correlate<NVT, NJB>), dim3(blocks), dim3(kWave * a.nut), shmem, stream, a);
+    hipLaunchKernelGGL((int8Piped<NVT, NJB>), dim3(blocks), dim3(kWave * kPipedWaves), shmem, stream, a);
+}
+
+// kind: 0 the one-frame kernel with full bands, 1 the same with the edge row and column waves,
+// 2 pipelined
+template <int NVT, int NJB>
+void launchOne(const Int8Args &a, int kind, int blocks, size_t shmem, hipStream_t stream)
+{
+    switch (kind) {
+    case 0: launchSimple<NVT, NJB, 0>(a, blocks, shmem, stream); break;
+    case 1: launchSimple<NVT, NJB, 2>(a, blocks, shmem, stream); break;
+    default: launchPiped<NVT, NJB>(a, blocks, shmem, stream); break;
+    }
 }
 
 template <int NJB>
-bool dispatchNvt(int nvt, const Int8Args &a, bool piped, int blocks, size_t shmem, hipStream_t stream)
+bool dispatchNvt(int nvt, const Int8Args &a, int kind, int blocks, size_t shmem, hipStream_t stream)
 {
     switch (nvt) {
-    case 1: launchOne<1, NJB>(a, piped, blocks, shmem, stream); return true;
-    case 2: launchOne<2, NJB>(a, piped, blocks, shmem, stream); return true;
-    case 3: launchOne<3, NJB>(a, piped, blocks, shmem, stream); return true;
-    case 4: launchOne<4, NJB>(a, piped, blocks, shmem, stream); return true;
-    case 5: launchOne<5, NJB>(a, piped, blocks, shmem, stream); return true;
-    case 6: launchOne<6, NJB>(a, piped, blocks, shmem, stream); return true;
-    case 7: launchOne<7, NJB>(a, piped, blocks, shmem, stream); return true;
-    case 8: launchOne<8, NJB>(a, piped, blocks, shmem, stream); return true;
-    case 9: launchOne<9, NJB>(a, piped, blocks, shmem, stream); return true;
+    case 1: launchOne<1, NJB>(a, kind, blocks, shmem, stream); return true;
+    case 2: launchOne<2, NJB>(a, kind, blocks, shmem, stream); return true;
+    case 3: launchOne<3, NJB>(a, kind, blocks, shmem, stream); return true;
+    case 4: launchOne<4, NJB>(a, kind, blocks, shmem, stream); return true;
+    case 5: launchOne<5, NJB>(a, kind, blocks, shmem, stream); return true;
+    case 6: launchOne<6, NJB>(a, kind, blocks, shmem, stream); return true;
+    case 7: launchOne<7, NJB>(a, kind, blocks, shmem, stream); return true;
+    case 8: launchOne<8, NJB>(a, kind, blocks, shmem, stream); return true;
+    case 9: launchOne<9, NJB>(a, kind, blocks, shmem, stream); return true;
     default: return false;
     }
 }
@@ -540,11 +656,13 @@ Int8Layout int8Layout(const Geometry &g)
     l.cstride = 16 * cs;
     l.crows = 16 * l.nut + 4 * l.ni4;
     const size_t tb = size_t(4 * l.ni4) * l.tstride, cb = size_t(l.crows) * l.cstride;
-    const size_t sb = sizeof(int) * size_t((g.M * g.M + 3) & ~3), pb = sizeof(WaveBest) * l.nut;
+    // (one WaveBest per wave: nut, or nut + 1 with the edge row and column in two waves)
+    const size_t sb = sizeof(int) * size_t((g.M * g.M + 3) & ~3), pb = sizeof(WaveBest) * (l.nut + 1);
     l.shmem = tb + cb + sb + pb;
     l.alias = l.shmem > 80 * 1024;   // two blocks per CU need at most 80 KB each; else share chip and surface
     if (l.alias) l.shmem = tb + (cb > sb ? cb : sb) + pb;
     l.supported = l.njb > 0 && l.nvt <= 9 && l.shmem <= 160 * 1024;
+    l.edge = g.M % 16 == 1 && g.M > 1;
     // the pipelined kernel: two chips, two raw surfaces, two sets of helper results
     const int nh = kPipedWaves - l.nut;
     l.pipedShmem = size_t(4 * l.ni4) * l.tstride + 2 * size_t(l.crows) * l.cstride
@@ -571,14 +689,16 @@ void launchInt8(Int8Args a, const Int8Layout &l, Int8Variant variant, hipStream_
     a.nfb = (a.fg.n + a.fpb - 1) / a.fpb;
     const int blocks = a.g.nwin * a.nfb;
     const bool piped = variant == Int8Variant::Piped && l.pipedOk;
+    const bool edge = !piped && variant != Int8Variant::Padded && l.edge;
+    const int kind = piped ? 2 : edge ? 1 : 0, nvt = edge ? l.nvt - 1 : l.nvt;
     a.alias = l.alias ? 1 : 0;
     const size_t shmem = piped ? l.pipedShmem : l.shmem;
     bool ok = false;
     switch (l.njb) {
-    case 2: ok = dispatchNvt<2>(l.nvt, a, piped, blocks, shmem, stream); break;
-    case 3: ok = dispatchNvt<3>(l.nvt, a, piped, blocks, shmem, stream); break;
-    case 5: ok = dispatchNvt<5>(l.nvt, a, piped, blocks, shmem, stream); break;
-    case 9: ok = dispatchNvt<9>(l.nvt, a, piped, blocks, shmem, stream); break;
+    case 2: ok = dispatchNvt<2>(nvt, a, kind, blocks, shmem, stream); break;
+    case 3: ok = dispatchNvt<3>(nvt, a, kind, blocks, shmem, stream); break;
+    case 5: ok = dispatchNvt<5>(nvt, a, kind, blocks, shmem, stream); break;
+    case 9: ok = dispatchNvt<9>(nvt, a, kind, blocks, shmem, stream); break;
     default: break;
     }
     if (!ok) throw std::invalid_argument("framecor: no int8 kernel for this window and search range");

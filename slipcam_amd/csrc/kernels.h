@@ -35,13 +35,15 @@ struct Int8Layout {
     size_t shmem;         // the one-frame-at-a-time kernel
     bool alias;           // its raw surface over its chip (large windows)
     bool supported;
+    bool edge;            // M = 16 c + 1: c core bands plus a wave for the edge row and one for the edge column
     size_t pipedShmem;    // the pipelined producer/helper kernel
     bool pipedOk;
 };
 
-// Auto: the one-frame-at-a-time kernel (two blocks per CU); Piped: the producer/helper pipeline where its
-// buffers fit in LDS
-enum class Int8Variant { Auto = 0, Simple = 1, Piped = 2 };
+// Auto and Simple: the one-frame-at-a-time kernel (two blocks per CU), with the edge row and the edge
+// column in a wave each where M = 16 c + 1; Piped: the producer/helper pipeline where its buffers fit
+// in LDS; Padded: the one-frame kernel with ceil(M / 16) full bands (no edge waves)
+enum class Int8Variant { Auto = 0, Simple = 1, Piped = 2, Padded = 3 };
 
 Int8Layout int8Layout(const Geometry &g);
 void launchInt8(Int8Args a, const Int8Layout &l, Int8Variant variant, hipStream_t stream);

@@ -204,8 +204,10 @@ class Correlator:
         self._ready = True
 
     def set_variant(self, variant):
-        """The int8 kernel: 'auto', 'simple' (one frame at a time) or 'piped' (producer/helper pipeline)"""
-        _check(_lib.framecor_set_variant(self._c, {'auto': 0, 'simple': 1, 'piped': 2}[variant]))
+        """The int8 kernel: 'auto', 'simple' (one frame at a time), 'piped' (producer/helper pipeline) or
+        'padded' (one frame at a time, full 16-row bands even where only an edge row and column remain);
+        every variant gives the same offsets, peaks and covariances, the SNR to rounding"""
+        _check(_lib.framecor_set_variant(self._c, {'auto': 0, 'simple': 1, 'piped': 2, 'padded': 3}[variant]))
 
     def set_timing(self, enable=True):
         """Record the GPU time of each stage of the next correlate calls (hipEvents on their stream)"""

@@ -207,8 +207,9 @@ def test_grouping_and_device_calls_agree(gpu, fault, method):
 
 @pytest.mark.parametrize('workload', ['small', 'fault'])
 def test_int8_kernel_variants_agree(gpu, small, fault, workload):
-    """The pipelined producer/helper kernel and the one-frame-at-a-time kernel: the same outputs (the
-    SNR sums are reduced in another order: equal to 1e-12)"""
+    """The one-frame-at-a-time kernel with the edge row and column waves (search 48 and 16: M = 16 c + 1),
+    the same with full 16-row bands, and the pipelined producer/helper kernel: the same offsets, peaks and
+    covariances, the SNR to 1e-12 (its sums are split over the helper waves)"""
     if workload == 'small':
         d, down, across, _, gross = small
         h, w, W, R, frames, g = 256, 320, d['window'], d['search'], d['frames'], gross
@@ -216,14 +217,16 @@ def test_int8_kernel_variants_agree(gpu, small, fault, workload):
         d, down, across, _ = fault
         h, w, W, R, frames, g = 720, 1280, 64, 48, d['frames'], None
     out = {}
-    for variant in ('simple', 'piped'):
+    for variant in ('simple', 'piped', 'padded'):
         cor = framecor.Correlator(h, w, down, across, window=W, search=R, method='int8')
         cor.set_variant(variant)
         cor.set_reference(d['ref'])
         out[variant] = cor.correlate(frames, gross=g)
-    a, b = out['simple'], out['piped']
-    for key in ('offsets', 'peak', 'cov'):
-        assert np.array_equal(a[key], b[key], equal_nan=True), key
+    a = out['simple']
     ok = a['snr'] > 0
-    assert np.array_equal(ok, b['snr'] > 0)
-    assert np.abs(a['snr'][ok] / b['snr'][ok] - 1).max() < 1e-12
+    for other in ('padded', 'piped'):   # the same surface, its SNR sums split over other waves
+        b = out[other]
+        for key in ('offsets', 'peak', 'cov'):
+            assert np.array_equal(a[key], b[key], equal_nan=True), (other, key)
+        assert np.array_equal(ok, b['snr'] > 0), other
+        assert np.abs(a['snr'][ok] / b['snr'][ok] - 1).max() < 1e-12, other
