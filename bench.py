@@ -54,7 +54,7 @@ def time_method(torch, framecor, d, dev, method, steps, warmup):
     elapsed = time.perf_counter() - t0
     # the same steps again with per-stage hipEvents (the stream of the calls)
     cor.set_timing(True)
-    stages = dict(summed_area=0.0, box_sums=0.0, correlate=0.0, launches=0)
+    stages = dict(box_sums=0.0, correlate=0.0, launches=0)
     for _ in range(steps):
         cor.correlate(dev)
         t = cor.timing()
@@ -70,7 +70,7 @@ def time_method(torch, framecor, d, dev, method, steps, warmup):
     return dict(cor=cor, nwin=nwin, n=n, ms_per_step=ms_step, ms_per_frame=ms_step / n,
                 frames_per_s=n * steps / elapsed, windows_per_s=nwin * n * steps / elapsed,
                 set_reference_ms=set_ref_ms, host_api_ms_per_frame=host_ms / n,
-                stage_ms_per_frame={k: stages[k] / (steps * n) for k in ('summed_area', 'box_sums', 'correlate')},
+                stage_ms_per_frame={k: stages[k] / (steps * n) for k in ('box_sums', 'correlate')},
                 kernel_ms_per_launch=stages['correlate'] / max(stages['launches'], 1),
                 frames_per_launch=n * steps / max(stages['launches'], 1))
 

@@ -125,7 +125,7 @@ int framecor_set_variant(framecor_correlator *c, int variant);
 /* per-stage GPU timing (no counterpart in the reference; for benchmarks): with enable 1, the
  * correlate calls record hipEvents on their stream around each stage; framecor_get_timing waits for
  * the last call's events and writes the milliseconds of its stages, summed over its groups of frames:
- * ms[0] summed-area tables, ms[1] box sums, ms[2] the correlation kernel; and *launches, the number of
+ * ms[0] the box-sum maps, ms[1] the correlation kernel; and *launches, the number of
  * correlation-kernel launches */
 int framecor_set_timing(framecor_correlator *c, int enable);
 int framecor_get_timing(framecor_correlator *c, float *ms, int *launches);
@@ -135,6 +135,14 @@ int framecor_get_timing(framecor_correlator *c, float *ms, int *launches);
  * Enqueued on stream (NULL: the null stream), not synchronized. */
 int framecor_summed_area_device(const uint8_t *frames, int n, int height, int width, int64_t *sat,
                                 int64_t *sat2, void *stream);
+
+/* the W x W box sums of n frames (device memory) at every window position: box and box2,
+ * n x (height - window + 1) x (width - window + 1) int32, the sums of the pixels and of their squares;
+ * what boxSum (cxx/surface.h:21-25) reads from the summed-area tables, computed here straight from
+ * the pixels (the correlator's first stage). window 1 .. 181. Enqueued on stream (NULL: the null
+ * stream), not synchronized. */
+int framecor_box_sums_device(const uint8_t *frames, int n, int height, int width, int window, int32_t *box,
+                             int32_t *box2, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,410 +1,13 @@
-// capi.hip -- the C ABI of libframecor_hip.so (include/framecor_hip.h): the correlator object, its
-// device memory, set_reference and the per-group launch sequence
-//
-// Per group of frames: the summed-area tables (satRows, satCols), the W x W box-sum maps (boxSums),
-// then one correlation launch (int8Correlate or fftCorrelate) that writes the outputs. The reference
-// windows are prepared on the host in set_reference exactly as cxx/common.cpp:30-48 (mean in FP64,
-// d = float(px - mean), norm = float(sqrt(sum d^2)) in row-major order) so that the validity of a
-// window (norm > 0) and the FFT form's normalization use the reference's norm bits.
+// capi.hip -- the C ABI of libframecor_hip.so (include/framecor_hip.h) over the correlator object
+// of correlator.h
 
-#include <algorithm>
-#include <cmath>
-#include <cstring>
-#include <memory>
-#include <new>
-#include <stdexcept>
-#include <string>
-#include <thread>
-#include <vector>
-
-#include "common.h"
-#include "framecor_hip.h"
-#include "kernels.h"
+#include "correlator.h"
 
 #ifndef FRAMECOR_VERSION
 #define FRAMECOR_VERSION "0.1.0"
 #endif
 
 using namespace framecor;
-
-namespace {
-
-thread_local std::string lastError;
-
-int fail(int status, const std::string &msg)
-{
-    lastError = msg;
-    return status;
-}
-
-template <class F>
-int guarded(F &&f)
-{
-    try {
-        f();
-        lastError.clear();
-        return FRAMECOR_OK;
-    }
-    catch (const std::invalid_argument &e) {
-        return fail(FRAMECOR_EINVAL, e.what());
-    }
-    catch (const std::bad_alloc &) {
-        return fail(FRAMECOR_ENOMEM, "framecor: out of memory");
-    }
-    catch (const std::exception &e) {
-        return fail(FRAMECOR_ERUNTIME, e.what());
-    }
-    catch (...) {
-        return fail(FRAMECOR_ERUNTIME, "framecor: unknown error");
-    }
-}
-
-[[noreturn]] void invalid(const std::string &msg) { throw std::invalid_argument("framecor: " + msg); }
-
-// device memory owned by a correlator
-struct DeviceBuffer {
-    void *ptr = nullptr;
-    size_t bytes = 0;
-    DeviceBuffer() = default;
-    DeviceBuffer(const DeviceBuffer &) = delete;
-    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
-    ~DeviceBuffer() { release(); }
-    void release()
-    {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-    void ensure(size_t n)
-    {
-        if (n <= bytes && ptr) return;
-        release();
-        const hipError_t e = hipMalloc(&ptr, std::max<size_t>(n, 16));
-        if (e == hipErrorOutOfMemory) {
-            (void)hipGetLastError();
-            ptr = nullptr;
-            throw std::bad_alloc();
-        }
-        FC_HIP_CHECK(e);
-        bytes = n;
-    }
-    template <class T>
-    T *as() const { return static_cast<T *>(ptr); }
-};
-
-template <class F>
-void parallelFor(int n, F &&body)
-{
-    const int nt = std::max(1, std::min<int>(16, int(std::thread::hardware_concurrency())));
-    if (n < 64 || nt == 1) {
-        for (int i = 0; i < n; i++) body(i);
-        return;
-    }
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nt; t++)
-        pool.emplace_back([&, t] {
-            for (int i = t; i < n; i += nt) body(i);
-        });
-    for (auto &th : pool) th.join();
-}
-
-} // namespace
-
-struct framecor_correlator {
-    Geometry g{};
-    std::vector<int> down, across;
-    int method = FRAMECOR_METHOD_FFT;
-    int batch = 4096;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool ready = false;
-    int maxGroup = 1;
-    Int8Layout i8{};
-    Int8Variant variant = Int8Variant::Auto;
-    FftPlan plan{};
-
-    // per window, written by set_reference
-    DeviceBuffer dDown, dAcross, dNorm, dTsum, dTvar, dTmpl, dSpectra, dTwiddle;
-    // per group of frames
-    DeviceBuffer dSat, dSat2, dBox, dBox2, dScreen;
-    int groupCap = 0;
-    // host-call staging
-    DeviceBuffer dFrames, dGross, dOffset, dPeak, dSnr, dCov;
-    // per-stage timing: four events per group of the last call
-    bool timing = false;
-    std::vector<hipEvent_t> events;
-    int eventsUsed = 0;
-
-    ~framecor_correlator()
-    {
-        for (hipEvent_t e : events) (void)hipEventDestroy(e);
-        if (stream) {
-            (void)hipSetDevice(device);
-            (void)hipStreamDestroy(stream);
-        }
-    }
-
-    WindowData windowData() const
-    {
-        WindowData wd;
-        wd.down = dDown.as<int>();
-        wd.across = dAcross.as<int>();
-        wd.norm = dNorm.as<float>();
-        wd.tsum = dTsum.as<int64_t>();
-        wd.tvar = dTvar.as<int64_t>();
-        return wd;
-    }
-
-    size_t bytesPerFrame() const
-    {
-        return size_t(g.satSize) * 16 + size_t(g.boxSize) * 16;
-    }
-
-    void ensureGroup(int n)
-    {
-        const int want = std::min(n, maxGroup);
-        if (want <= groupCap) return;
-        dSat.ensure(size_t(want) * g.satSize * sizeof(int64_t));
-        dSat2.ensure(size_t(want) * g.satSize * sizeof(int64_t));
-        dBox.ensure(size_t(want) * g.boxSize * sizeof(int32_t));
-        dBox2.ensure(size_t(want) * g.boxSize * sizeof(int32_t));
-        if (method == FRAMECOR_METHOD_INT8) dScreen.ensure(size_t(want) * g.boxSize * sizeof(int2));
-        groupCap = want;
-    }
-
-    void mark(hipStream_t s)
-    {
-        if (!timing) return;
-        if (eventsUsed == int(events.size())) {
-            hipEvent_t e;
-            FC_HIP_CHECK(hipEventCreate(&e));
-            events.push_back(e);
-        }
-        FC_HIP_CHECK(hipEventRecord(events[eventsUsed++], s));
-    }
-
-    // one group of frames, all pointers on the device
-    void runGroup(const uint8_t *frames, int n, const int32_t *gross, float *offset, float *peak, float *snr,
-                  float *cov, hipStream_t s)
-    {
-        mark(s);
-        launchSummedArea(frames, n, g.h, g.w, dSat.as<int64_t>(), dSat2.as<int64_t>(), s);
-        mark(s);
-        launchBoxSums(dSat.as<int64_t>(), dSat2.as<int64_t>(), n, g.h, g.w, g.W, dBox.as<int32_t>(),
-                      dBox2.as<int32_t>(), method == FRAMECOR_METHOD_INT8 ? dScreen.as<int2>() : nullptr, s);
-        mark(s);
-        FrameGroup fg;
-        fg.frames = frames;
-        fg.gross = gross;
-        fg.sat = dSat.as<int64_t>();
-        fg.box = dBox.as<int32_t>();
-        fg.box2 = dBox2.as<int32_t>();
-        fg.screen = method == FRAMECOR_METHOD_INT8 ? dScreen.as<int2>() : nullptr;
-        fg.n = n;
-        fg.offset = offset;
-        fg.peak = peak;
-        fg.snr = snr;
-        fg.cov = cov;
-        if (method == FRAMECOR_METHOD_INT8) {
-            Int8Args a{};
-            a.g = g;
-            a.wd = windowData();
-            a.fg = fg;
-            a.tmpl = dTmpl.as<uint8_t>();
-            launchInt8(a, i8, variant, s);
-        }
-        else {
-            FftArgs a{};
-            a.g = g;
-            a.wd = windowData();
-            a.fg = fg;
-            a.spectra = dSpectra.as<float2>();
-            a.twiddle = dTwiddle.as<float2>();
-            a.plan = plan;
-            a.np = (g.L + 1) / 2;
-            a.H = g.L / 2 + 1;
-            a.rs = 2 * a.H;
-            launchFftCorrelate(a, s);
-        }
-        mark(s);
-    }
-
-    void correlateDevice(const uint8_t *frames, int n, const int32_t *gross, float *offset, float *peak,
-                         float *snr, float *cov, hipStream_t s)
-    {
-        if (!ready) throw std::runtime_error("framecor: set_reference first");
-        if (n < 0) invalid("the number of frames must not be negative");
-        eventsUsed = 0;
-        if (n == 0) return;
-        FC_HIP_CHECK(hipSetDevice(device));
-        ensureGroup(n);
-        for (int f0 = 0; f0 < n; f0 += groupCap) {
-            const int nf = std::min(groupCap, n - f0);
-            const size_t q0 = size_t(f0) * g.nwin;
-            runGroup(frames + size_t(f0) * g.pixels, nf, gross ? gross + 2 * q0 : nullptr, offset + 2 * q0,
-                     peak + q0, snr + q0, cov + 3 * q0, s);
-        }
-    }
-
-    void correlateHost(const uint8_t *frames, int n, const int32_t *gross, float *offset, float *peak, float *snr,
-                       float *cov)
-    {
-        if (!ready) throw std::runtime_error("framecor: set_reference first");
-        if (n < 0) invalid("the number of frames must not be negative");
-        eventsUsed = 0;
-        if (n == 0) return;
-        FC_HIP_CHECK(hipSetDevice(device));
-        ensureGroup(n);
-        const int G = groupCap;
-        dFrames.ensure(size_t(G) * g.pixels);
-        dOffset.ensure(size_t(G) * g.nwin * 2 * sizeof(float));
-        dPeak.ensure(size_t(G) * g.nwin * sizeof(float));
-        dSnr.ensure(size_t(G) * g.nwin * sizeof(float));
-        dCov.ensure(size_t(G) * g.nwin * 3 * sizeof(float));
-        if (gross) dGross.ensure(size_t(G) * g.nwin * 2 * sizeof(int32_t));
-        for (int f0 = 0; f0 < n; f0 += G) {
-            const int nf = std::min(G, n - f0);
-            const size_t q0 = size_t(f0) * g.nwin, pairs = size_t(nf) * g.nwin;
-            FC_HIP_CHECK(hipMemcpyAsync(dFrames.ptr, frames + size_t(f0) * g.pixels, size_t(nf) * g.pixels,
-                                        hipMemcpyHostToDevice, stream));
-            if (gross)
-                FC_HIP_CHECK(hipMemcpyAsync(dGross.ptr, gross + 2 * q0, pairs * 2 * sizeof(int32_t),
-                                            hipMemcpyHostToDevice, stream));
-            runGroup(dFrames.as<uint8_t>(), nf, gross ? dGross.as<int32_t>() : nullptr, dOffset.as<float>(),
-                     dPeak.as<float>(), dSnr.as<float>(), dCov.as<float>(), stream);
-            FC_HIP_CHECK(hipMemcpyAsync(offset + 2 * q0, dOffset.ptr, pairs * 2 * sizeof(float), hipMemcpyDeviceToHost, stream));
-            FC_HIP_CHECK(hipMemcpyAsync(peak + q0, dPeak.ptr, pairs * sizeof(float), hipMemcpyDeviceToHost, stream));
-            FC_HIP_CHECK(hipMemcpyAsync(snr + q0, dSnr.ptr, pairs * sizeof(float), hipMemcpyDeviceToHost, stream));
-            FC_HIP_CHECK(hipMemcpyAsync(cov + 3 * q0, dCov.ptr, pairs * 3 * sizeof(float), hipMemcpyDeviceToHost, stream));
-            FC_HIP_CHECK(hipStreamSynchronize(stream));
-        }
-    }
-
-    DiffArgs diffArgs(const float *frames, int n, const int32_t *gross, const float *shifts, float *ncc) const
-    {
-        if (!ready) throw std::runtime_error("framecor: set_reference first");
-        if (n < 0) invalid("the number of frames must not be negative");
-        DiffArgs d{};
-        d.g = g;
-        d.wd = windowData();
-        d.tmpl = dTmpl.as<uint8_t>();
-        d.frames = frames;
-        d.n = n;
-        d.gross = gross;
-        d.shifts = shifts;
-        d.ncc = ncc;
-        return d;
-    }
-
-    void setReference(const uint8_t *frame)
-    {
-        FC_HIP_CHECK(hipSetDevice(device));
-        const int W = g.W, nwin = g.nwin, n = W * W;
-        std::vector<uint8_t> tmpl(size_t(nwin) * n);
-        std::vector<float> norm(nwin), templ;
-        std::vector<int64_t> tsum(nwin), tvar(nwin);
-        const bool fft = method == FRAMECOR_METHOD_FFT;
-        if (fft) templ.resize(size_t(nwin) * n);
-        parallelFor(nwin, [&](int k) {
-            const int y0 = down[k / g.nx], x0 = across[k % g.nx];
-            uint8_t *t = tmpl.data() + size_t(k) * n;
-            int64_t s = 0, s2 = 0;
-            for (int i = 0; i < W; i++)
-                for (int j = 0; j < W; j++) {
-                    const int64_t px = frame[size_t(y0 + i) * g.w + x0 + j];
-                    t[i * W + j] = uint8_t(px);
-                    s += px;
-                    s2 += px * px;
-                }
-            tsum[k] = s;
-            tvar[k] = int64_t(n) * s2 - s * s;
-            // the zero-mean window and its norm, as cxx/common.cpp:30-48
-            const double mean = double(s) / (double(W) * W);
-            double sq = 0.0;
-            for (int i = 0; i < n; i++) {
-                const float d = float(t[i] - mean);
-                if (fft) templ[size_t(k) * n + i] = d;
-                sq += double(d) * d;
-            }
-            norm[k] = float(std::sqrt(sq));
-        });
-        dTmpl.ensure(tmpl.size());
-        dNorm.ensure(sizeof(float) * nwin);
-        dTsum.ensure(sizeof(int64_t) * nwin);
-        dTvar.ensure(sizeof(int64_t) * nwin);
-        FC_HIP_CHECK(hipMemcpyAsync(dTmpl.ptr, tmpl.data(), tmpl.size(), hipMemcpyHostToDevice, stream));
-        FC_HIP_CHECK(hipMemcpyAsync(dNorm.ptr, norm.data(), sizeof(float) * nwin, hipMemcpyHostToDevice, stream));
-        FC_HIP_CHECK(hipMemcpyAsync(dTsum.ptr, tsum.data(), sizeof(int64_t) * nwin, hipMemcpyHostToDevice, stream));
-        FC_HIP_CHECK(hipMemcpyAsync(dTvar.ptr, tvar.data(), sizeof(int64_t) * nwin, hipMemcpyHostToDevice, stream));
-        if (fft) {
-            DeviceBuffer dTempl;
-            dTempl.ensure(sizeof(float) * templ.size());
-            FC_HIP_CHECK(hipMemcpyAsync(dTempl.ptr, templ.data(), sizeof(float) * templ.size(), hipMemcpyHostToDevice, stream));
-            FftArgs a{};
-            a.g = g;
-            a.wd = windowData();
-            a.twiddle = dTwiddle.as<float2>();
-            a.templ = dTempl.as<float>();
-            a.spectraOut = dSpectra.as<float2>();
-            a.plan = plan;
-            a.np = (g.L + 1) / 2;
-            a.H = g.L / 2 + 1;
-            a.rs = 2 * a.H;
-            launchFftReference(a, stream);
-            FC_HIP_CHECK(hipStreamSynchronize(stream));
-        }
-        else {
-            FC_HIP_CHECK(hipStreamSynchronize(stream));
-        }
-        ready = true;
-    }
-};
-
-namespace {
-
-// the reference's parameter checks, with its messages (cxx/common.cpp:15-28)
-void checkParams(int height, int width, const int *down, int ndown, const int *across, int nacross, int window,
-                 int search, int batch, int stat)
-{
-    if (height <= 0 || width <= 0) invalid("the frame size must be positive");
-    if (window < 3) invalid("the window must be at least 3 pixels");
-    if (search < 1) invalid("the half search range must be at least 1 pixel");
-    if (ndown <= 0 || nacross <= 0 || !down || !across) invalid("no windows");
-    for (int i = 0; i < ndown; i++)
-        if (down[i] < 0 || down[i] + window > height)
-            invalid("a window is outside the frame (down " + std::to_string(down[i]) + ")");
-    for (int i = 0; i < nacross; i++)
-        if (across[i] < 0 || across[i] + window > width)
-            invalid("a window is outside the frame (across " + std::to_string(across[i]) + ")");
-    if (stat < 3) invalid("the SNR window must be at least 3 pixels");
-    if (batch < 1) invalid("the batch must be at least 1 window");
-}
-
-// the reference's checks, then what the method supports
-void checkAll(int height, int width, const int *down, int ndown, const int *across, int nacross, int window,
-              int search, int method, int batch, int stat)
-{
-    checkParams(height, width, down, ndown, across, nacross, window, search, batch, stat);
-    if (method != FRAMECOR_METHOD_FFT && method != FRAMECOR_METHOD_INT8) invalid("the method must be fft or int8");
-    if (window > 181) invalid("the window must be at most 181 pixels (exact int32 box sums)");
-    Geometry g{};
-    g.W = window;
-    g.R = search;
-    g.L = window + 2 * search;
-    g.M = 2 * search + 1;
-    if (method == FRAMECOR_METHOD_INT8) {
-        if (!int8Layout(g).supported)
-            invalid("the int8 method supports windows up to 129 pixels and search ranges up to 71 pixels");
-    }
-    else {
-        FftPlan plan;
-        if (!fftPlanFor(g.L, plan))
-            invalid("the fft method needs a search window (window + 2 search = " + std::to_string(g.L)
-                    + ") whose half spectrum fits in LDS (at most 192) and whose prime factors are at most 31");
-    }
-}
-
-} // namespace
 
 extern "C" {
 
@@ -427,7 +30,7 @@ int framecor_has_gpu(void)
     return count > 0;
 }
 
-const char *framecor_last_error(void) { return lastError.c_str(); }
+const char *framecor_last_error(void) { return lastError().c_str(); }
 
 int framecor_create(int height, int width, const int *down, int ndown, const int *across, int nacross, int window,
                     int search, int method, int batch, int stat_size, int device, framecor_correlator **out)
@@ -435,62 +38,8 @@ int framecor_create(int height, int width, const int *down, int ndown, const int
     if (!out) return fail(FRAMECOR_EINVAL, "framecor: no output handle");
     *out = nullptr;
     return guarded([&] {
-        checkAll(height, width, down, ndown, across, nacross, window, search, method, batch, stat_size);
         auto c = std::make_unique<framecor_correlator>();
-        Geometry &g = c->g;
-        g.h = height;
-        g.w = width;
-        g.W = window;
-        g.R = search;
-        g.L = window + 2 * search;
-        g.M = 2 * search + 1;
-        g.ny = ndown;
-        g.nx = nacross;
-        g.nwin = ndown * nacross;
-        g.stat = stat_size;
-        g.bh = height - window + 1;
-        g.bw = width - window + 1;
-        g.pixels = int64_t(height) * width;
-        g.satSize = int64_t(height + 1) * (width + 1);
-        g.boxSize = int64_t(g.bh) * g.bw;
-        c->down.assign(down, down + ndown);
-        c->across.assign(across, across + nacross);
-        c->method = method;
-        c->batch = batch;
-        c->device = device;
-        if (method == FRAMECOR_METHOD_INT8)
-            c->i8 = int8Layout(g);
-        else
-            fftPlanFor(g.L, c->plan);
-        // groups of frames: at most 2 GiB of tables
-        const size_t perFrame = c->bytesPerFrame();
-        c->maxGroup = int(std::max<size_t>(1, std::min<size_t>(256, (size_t(2) << 30) / perFrame)));
-
-        // the device
-        int count = 0;
-        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
-            (void)hipGetLastError();
-            throw std::runtime_error("framecor: no GPU is available");
-        }
-        if (device < 0 || device >= count) invalid("no GPU " + std::to_string(device));
-        FC_HIP_CHECK(hipSetDevice(device));
-        FC_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        c->dDown.ensure(sizeof(int) * ndown);
-        c->dAcross.ensure(sizeof(int) * nacross);
-        FC_HIP_CHECK(hipMemcpy(c->dDown.ptr, down, sizeof(int) * ndown, hipMemcpyHostToDevice));
-        FC_HIP_CHECK(hipMemcpy(c->dAcross.ptr, across, sizeof(int) * nacross, hipMemcpyHostToDevice));
-        if (method == FRAMECOR_METHOD_FFT) {
-            const int L = g.L;
-            std::vector<float2> tw(L);
-            for (int k = 0; k < L; k++) {
-                const double ang = -2.0 * M_PI * double(k) / double(L);
-                tw[k] = make_float2(float(std::cos(ang)), float(std::sin(ang)));
-            }
-            c->dTwiddle.ensure(sizeof(float2) * L);
-            FC_HIP_CHECK(hipMemcpy(c->dTwiddle.ptr, tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
-            const size_t np = (L + 1) / 2, rs = 2 * (L / 2 + 1);
-            c->dSpectra.ensure(sizeof(float2) * np * rs * size_t(g.nwin));
-        }
+        c->init(height, width, down, ndown, across, nacross, window, search, method, batch, stat_size, device);
         *out = c.release();
     });
 }
@@ -596,13 +145,13 @@ int framecor_get_timing(framecor_correlator *c, float *ms, int *launches)
 {
     if (!c || !ms || !launches) return fail(FRAMECOR_EINVAL, "framecor: null argument");
     return guarded([&] {
-        ms[0] = ms[1] = ms[2] = 0.0f;
-        *launches = c->eventsUsed / 4;
+        ms[0] = ms[1] = 0.0f;
+        *launches = c->eventsUsed / 3;
         if (c->eventsUsed == 0) return;
         FC_HIP_CHECK(hipSetDevice(c->device));
         FC_HIP_CHECK(hipEventSynchronize(c->events[c->eventsUsed - 1]));
-        for (int i = 0; i + 3 < c->eventsUsed; i += 4)
-            for (int s = 0; s < 3; s++) {
+        for (int i = 0; i + 2 < c->eventsUsed; i += 3)
+            for (int s = 0; s < 2; s++) {
                 float t = 0.0f;
                 FC_HIP_CHECK(hipEventElapsedTime(&t, c->events[i + s], c->events[i + s + 1]));
                 ms[s] += t;
@@ -618,6 +167,19 @@ int framecor_summed_area_device(const uint8_t *frames, int n, int height, int wi
         if (n > 65535) invalid("at most 65535 frames per call");
         if (n > 0 && (!frames || !sat || !sat2)) invalid("null argument");
         launchSummedArea(frames, n, height, width, sat, sat2, static_cast<hipStream_t>(stream));
+    });
+}
+
+int framecor_box_sums_device(const uint8_t *frames, int n, int height, int width, int window, int32_t *box,
+                             int32_t *box2, void *stream)
+{
+    return guarded([&] {
+        if (n < 0 || height <= 0 || width <= 0) invalid("the frame size must be positive");
+        if (window < 1 || window > 181 || window > height || window > width)
+            invalid("the window must be 1 .. 181 pixels and fit in the frame");
+        if (n > 65535) invalid("at most 65535 frames per call");
+        if (n > 0 && (!frames || !box || !box2)) invalid("null argument");
+        launchBoxSums(frames, n, height, width, window, box, box2, nullptr, static_cast<hipStream_t>(stream));
     });
 }
 

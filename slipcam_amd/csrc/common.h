@@ -4,8 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <unordered_set>
 
 namespace framecor {
 
@@ -37,11 +39,50 @@ struct WindowData {
 struct FrameGroup {
     const uint8_t *frames;      // n x h x w
     const int *gross;           // n x nwin x 2, or null
-    const int64_t *sat;         // n x (h + 1) x (w + 1), the summed-area table of the pixels
-    const int32_t *box, *box2;  // n x bh x bw: W x W box sums of the pixels and of their squares
+    const int32_t *box, *box2;  // n x bh x bw: W x W box sums of the pixels (FFT form) and of their squares
     const int2 *screen;         // n x bh x bw: {box, 1 / sqrt(W^2 box2 - box^2) float bits} (int8 form)
     int n;
     float *offset, *peak, *snr, *cov;
+};
+
+constexpr int kXcds = 8;          // XCDs of an MI355X, each with its own L2
+constexpr int kQueueClips = 48;   // clips per launch of a work queue
+
+// The work queue of one tile shape over clips of a batch (a kernel argument). Items are (clip, window,
+// run of frames), numbered clip by clip and, within a clip, window-major (consecutive items share a
+// template): segment s covers items first[s] .. first[s + 1] - 1 of clip clip[s], nwin x nfb items of
+// fpb frames (the last run of a window may be shorter). The items are dealt into kXcds contiguous
+// ranges, one per XCD, each drained through its own counter (counters: kXcds ints, zero at launch);
+// a block takes items from its own XCD's range (blocks are dealt to the XCDs round-robin), then steals
+// from the others; -1 when every range is drained.
+struct WorkQueue {
+    int *counters;
+    int nitems, nseg;
+    int clip[kQueueClips], first[kQueueClips + 1], nfb[kQueueClips], fpb[kQueueClips], frames[kQueueClips];
+
+    __device__ int next(int block) const
+    {
+        const int home = block & (kXcds - 1);
+        for (int t = 0; t < kXcds; t++) {
+            const int x = (home + t) & (kXcds - 1);
+            const int lo = int(int64_t(nitems) * x / kXcds), hi = int(int64_t(nitems) * (x + 1) / kXcds);
+            if (lo >= hi || __atomic_load_n(&counters[x], __ATOMIC_RELAXED) >= hi - lo) continue;
+            const int i = atomicAdd(&counters[x], 1);
+            if (i < hi - lo) return lo + i;
+        }
+        return -1;
+    }
+
+    __device__ void locate(int it, int &c, int &k, int &fBegin, int &fEnd) const
+    {
+        int s = 0;
+        while (s + 1 < nseg && it >= first[s + 1]) s++;
+        c = clip[s];
+        const int local = it - first[s];
+        k = local / nfb[s];
+        fBegin = (local - k * nfb[s]) * fpb[s];
+        fEnd = min(fBegin + fpb[s], frames[s]);
+    }
 };
 
 // Blocks of a grid are dealt round-robin to the 8 XCDs of an MI355X, each with its own L2. Map the
@@ -116,3 +157,18 @@ __device__ inline void waveBest(T &v, int &i)
         if (e_ != hipSuccess)                                                                       \
             throw std::runtime_error(std::string("framecor: ") + #call + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+namespace framecor {
+
+// lets a kernel use up to the whole 160 KB of LDS of a CU (once per kernel and process)
+inline void allowLargeLds(const void *kernel)
+{
+    static std::mutex lock;
+    static std::unordered_set<const void *> done;
+    std::lock_guard<std::mutex> guard(lock);
+    if (!done.insert(kernel).second) return;
+    (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipGetLastError();
+}
+
+} // namespace framecor

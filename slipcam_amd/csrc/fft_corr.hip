@@ -1,7 +1,7 @@
 // fft_corr.hip -- the FP32 FFT form: one block per (window, frame), the half spectrum in LDS
 //
 // Per search window (cxx/cpu.cpp:117-167 of the reference, as one fused kernel): the L x L chip minus
-// its mean (from the summed-area table) is loaded into LDS as packed row pairs; forward row passes,
+// its mean (its exact pixel sum, taken while it is loaded) is loaded into LDS as packed row pairs; forward row passes,
 // the split into half spectra, forward column passes; the product with the conjugate of the cached
 // spectrum of the reference window (read from HBM once per window and frame, coalesced); inverse
 // column passes, the combine, inverse row passes over the M surface rows only; then, in LDS, the
@@ -33,6 +33,7 @@ struct FftShared {
     float *redv;    // waves
     int *redi;      // waves
     double *redd;   // waves
+    int *sums;      // waves: the chip sum
 };
 
 __device__ inline FftShared carve(unsigned char *lds, const FftArgs &a)
@@ -44,6 +45,7 @@ __device__ inline FftShared carve(unsigned char *lds, const FftArgs &a)
     s.pos = reinterpret_cast<int *>(s.redd + kFftThreads / kWave);
     s.redv = reinterpret_cast<float *>(s.pos + a.g.L);
     s.redi = reinterpret_cast<int *>(s.redv + kFftThreads / kWave);
+    s.sums = s.redi + kFftThreads / kWave;
     return s;
 }
 
@@ -89,15 +91,12 @@ __global__ void __launch_bounds__(kFftThreads) fftReference(FftArgs a)
     for (int i = tid; i < a.np * a.rs; i += kFftThreads) out[i] = s.Z[i];
 }
 
-__global__ void __launch_bounds__(kFftThreads) fftCorrelate(FftArgs a)
+// window k against frames fBegin .. fEnd - 1, by the whole block
+__device__ inline void fftWindow(const FftArgs &a, int k, int fBegin, int fEnd, unsigned char *lds)
 {
-    extern __shared__ __align__(16) unsigned char lds[];
     const FftShared s = carve(lds, a);
     const Geometry &g = a.g;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
-    const int logical = xcdRemap(blockIdx.x, gridDim.x);
-    const int k = logical / a.nfb, fblock = logical - k * a.nfb;
-    const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n);
     const int L = g.L, M = g.M, W = g.W;
     BlockSync sync;
 
@@ -119,20 +118,31 @@ __global__ void __launch_bounds__(kFftThreads) fftCorrelate(FftArgs a)
             if (tid == 0) invalidateOut(q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
             continue;
         }
-        // the chip minus its mean (the mean from the summed-area table, as cxx/cpu.cpp:130-137)
-        const int64_t *sat = a.fg.sat + f * g.satSize;
-        const int64_t st = g.w + 1;
-        const int64_t boxL = sat[(sy + L) * st + sx + L] - sat[sy * st + sx + L] - sat[(sy + L) * st + sx]
-            + sat[sy * st + sx];
-        const float mean = float(boxL) / (float(L) * float(L));
+        // the chip minus its mean (cxx/cpu.cpp:130-137: mean = float(box sum over L x L) / float(L^2),
+        // buf = px - mean). The pixels go to LDS as floats (exact) while the block sums them exactly in
+        // int32; then each becomes px - mean, the one FP32 rounding of the reference
         __syncthreads();   // the previous frame's epilogue read the surface
         const uint8_t *src = a.fg.frames + size_t(f) * g.pixels + size_t(sy) * g.w + sx;
+        int part = 0;
         for (int idx = tid; idx < L * L; idx += kFftThreads) {
             const int y = idx / L, x = idx - y * L;
-            zf[2 * ((y >> 1) * a.rs + x) + (y & 1)] = float(src[size_t(y) * g.w + x]) - mean;
+            const int px = src[size_t(y) * g.w + x];
+            part += px;
+            zf[2 * ((y >> 1) * a.rs + x) + (y & 1)] = float(px);
         }
         if (L & 1)   // the imaginary half of the last pair: zero
             for (int x = tid; x < L; x += kFftThreads) zf[2 * ((a.np - 1) * a.rs + x) + 1] = 0.0f;
+        part = waveSumI(part);
+        if (lane == 0) s.sums[wave] = part;
+        __syncthreads();
+        int boxL = 0;
+        for (int t = 0; t < kFftThreads / kWave; t++) boxL += s.sums[t];
+        const float mean = float(boxL) / (float(L) * float(L));
+        for (int idx = tid; idx < L * L; idx += kFftThreads) {
+            const int y = idx / L, x = idx - y * L;
+            float &z = zf[2 * ((y >> 1) * a.rs + x) + (y & 1)];
+            z = z - mean;
+        }
         __syncthreads();
 
         forward2d(s, a, tid);
@@ -209,6 +219,32 @@ __global__ void __launch_bounds__(kFftThreads) fftCorrelate(FftArgs a)
     }
 }
 
+__global__ void __launch_bounds__(kFftThreads) fftCorrelate(FftArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int logical = xcdRemap(blockIdx.x, gridDim.x);
+    const int k = logical / a.nfb, fblock = logical - k * a.nfb;
+    const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n);
+    fftWindow(a, k, fBegin, fEnd, lds);
+}
+
+// the persistent form over the work queue of one search-window size (every clip of a batch with this
+// L; clips[c]: clip c's arguments), as int8Queue
+__global__ void __launch_bounds__(kFftThreads) fftQueue(WorkQueue q, const FftArgs *__restrict__ clips)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int item[2];
+    for (int round = 0;; round++) {
+        if (threadIdx.x == 0) item[round & 1] = q.next(blockIdx.x);
+        __syncthreads();   // also: the previous item is over (its LDS may be rewritten)
+        const int it = __builtin_amdgcn_readfirstlane(item[round & 1]);
+        if (it < 0) break;
+        int c, k, fBegin, fEnd;
+        q.locate(it, c, k, fBegin, fEnd);
+        fftWindow(clips[c], k, fBegin, fEnd, lds);
+    }
+}
+
 void configure(const void *kern)
 {
     (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -242,7 +278,7 @@ size_t fftShmem(int L)
 {
     const int np = (L + 1) / 2, rs = 2 * (L / 2 + 1);
     return sizeof(float2) * size_t(np) * rs + sizeof(float2) * L + sizeof(double) * (kFftThreads / kWave)
-        + sizeof(int) * L + (sizeof(float) + sizeof(int)) * (kFftThreads / kWave);
+        + sizeof(int) * L + (sizeof(float) + 2 * sizeof(int)) * (kFftThreads / kWave);
 }
 
 void launchFftReference(const FftArgs &a, hipStream_t stream)
@@ -272,6 +308,24 @@ void launchFftCorrelate(FftArgs a, hipStream_t stream)
     if (blocks > 0x7fffffff) throw std::invalid_argument("framecor: too many windows x frames in one group");
     hipLaunchKernelGGL(fftCorrelate, dim3(unsigned(blocks)), dim3(kFftThreads), fftShmem(a.g.L), stream, a);
     FC_HIP_CHECK(hipGetLastError());
+}
+
+FftKernel fftQueueKernel(int L)
+{
+    FftKernel k{};
+    k.func = reinterpret_cast<const void *>(fftQueue);
+    k.threads = kFftThreads;
+    k.shmem = fftShmem(L);
+    allowLargeLds(k.func);
+    return k;
+}
+
+void launchFftQueue(const FftKernel &k, const WorkQueue &q, const FftArgs *clips, int blocks, hipStream_t stream)
+{
+    WorkQueue qa = q;
+    const FftArgs *c = clips;
+    void *args[] = {&qa, &c};
+    FC_HIP_CHECK(hipLaunchKernel(k.func, dim3(blocks), dim3(k.threads), args, k.shmem, stream));
 }
 
 } // namespace framecor

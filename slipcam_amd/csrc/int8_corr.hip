@@ -389,19 +389,15 @@ __device__ inline void helperFinish(const Int8Args &a, const Moments &mom, const
                      a.fg.snr, a.fg.cov);
 }
 
+// Window k against frames fBegin .. fEnd - 1, by the whole block (NVT + EW waves).
 // EW = 0: ceil(M / 16) = NVT full bands; EW = 2 (M = 16 NVT + 1): NVT core bands plus a wave for the edge
 // row and one for the edge column (EW = 1, both in one wave, makes that wave the critical path)
 template <int NVT, int NJB, int EW>
-__global__ void __launch_bounds__(64 * (NVT + EW))
-__attribute__((amdgpu_waves_per_eu(NVT + EW <= 8 ? 4 : 1))) int8Correlate(Int8Args a)
+__device__ inline void int8Window(const Int8Args &a, int k, int fBegin, int fEnd, unsigned char *lds)
 {
-    extern __shared__ __align__(16) unsigned char lds[];
     const Geometry &g = a.g;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6, nthreads = blockDim.x;
     const int nw = nthreads / kWave;
-    const int logical = xcdRemap(blockIdx.x, gridDim.x);
-    const int k = logical / a.nfb, fblock = logical - k * a.nfb;
-    const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n);
     const int W = g.W, M = g.M, L = g.L;
 
     // the raw surface after the chip, or (a.alias) over it: then the MFMAs of a frame must be over before
@@ -469,6 +465,37 @@ __attribute__((amdgpu_waves_per_eu(NVT + EW <= 8 ? 4 : 1))) int8Correlate(Int8Ar
     }
 }
 
+template <int NVT, int NJB, int EW>
+__global__ void __launch_bounds__(64 * (NVT + EW))
+__attribute__((amdgpu_waves_per_eu(NVT + EW <= 8 ? 4 : 1))) int8Correlate(Int8Args a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int logical = xcdRemap(blockIdx.x, gridDim.x);
+    const int k = logical / a.nfb, fblock = logical - k * a.nfb;
+    const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n);
+    int8Window<NVT, NJB, EW>(a, k, fBegin, fEnd, lds);
+}
+
+// The persistent form: a grid of resident blocks drains the work queue of a tile shape (every clip
+// of a batch with this window and search range; clips[c] holds clip c's arguments, fpb and nfb
+// included). Each block takes the next item of its XCD's queue, then steals from the other XCDs'.
+template <int NVT, int NJB, int EW>
+__global__ void __launch_bounds__(64 * (NVT + EW))
+__attribute__((amdgpu_waves_per_eu(NVT + EW <= 8 ? 4 : 1))) int8Queue(WorkQueue q, const Int8Args *__restrict__ clips)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int item[2];
+    for (int round = 0;; round++) {
+        if (threadIdx.x == 0) item[round & 1] = q.next(blockIdx.x);
+        __syncthreads();   // also: the previous item is over (its LDS may be rewritten)
+        const int it = __builtin_amdgcn_readfirstlane(item[round & 1]);
+        if (it < 0) break;
+        int c, k, fBegin, fEnd;
+        q.locate(it, c, k, fBegin, fEnd);
+        int8Window<NVT, NJB, EW>(clips[c], k, fBegin, fEnd, lds);
+    }
+}
+
 // ---- the pipelined kernel: producer and helper wavefronts
 //
 // Waves 0 .. P-1 (P = NVT bands) only run MFMAs; waves P .. are helpers. Frames are pipelined over two
@@ -484,17 +511,14 @@ __attribute__((amdgpu_waves_per_eu(NVT + EW <= 8 ? 4 : 1))) int8Correlate(Int8Ar
 // the next one, so one frame's MFMAs overlap the staging, screening and peak fitting of its neighbours.
 
 template <int NVT, int NJB>
-__global__ void __launch_bounds__(64 * kPipedWaves) int8Piped(Int8Args a)
+__device__ inline void int8PipedWindow(const Int8Args &a, int k, int fBegin, int fEnd, unsigned char *lds)
 {
-    extern __shared__ __align__(16) unsigned char lds[];
     const Geometry &g = a.g;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6, nthreads = blockDim.x;
     const int P = a.nut, nh = nthreads / kWave - P;
     const bool producer = wave < P;
     const int h = wave - P, htid = tid - kWave * P, hthreads = kWave * nh;
-    const int logical = xcdRemap(blockIdx.x, gridDim.x);
-    const int k = logical / a.nfb, fblock = logical - k * a.nfb;
-    const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n), F = fEnd - fBegin;
+    const int F = fEnd - fBegin;
     const int W = g.W, M = g.M, L = g.L;
 
     const int tbytes = 4 * a.ni4 * a.tstride, cbytes = a.crows * a.cstride, sints = (M * M + 3) & ~3;
@@ -570,58 +594,61 @@ __global__ void __launch_bounds__(64 * kPipedWaves) int8Piped(Int8Args a)
     }
 }
 
-template <int NVT, int NJB, int EW>
-void launchSimple(const Int8Args &a, int blocks, size_t shmem, hipStream_t stream)
+template <int NVT, int NJB>
+__global__ void __launch_bounds__(64 * kPipedWaves) int8Piped(Int8Args a)
 {
-    static bool configured = false;
-    if (!configured) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(int8Correlate<NVT, NJB, EW>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipGetLastError();
-        configured = true;
-    }
-    hipLaunchKernelGGL((int8Correlate<NVT, NJB, EW>), dim3(blocks), dim3(kWave * (NVT + EW)), shmem, stream, a);
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int logical = xcdRemap(blockIdx.x, gridDim.x);
+    const int k = logical / a.nfb, fblock = logical - k * a.nfb;
+    const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n);
+    int8PipedWindow<NVT, NJB>(a, k, fBegin, fEnd, lds);
 }
 
 template <int NVT, int NJB>
-void launchPiped(const Int8Args &a, int blocks, size_t shmem, hipStream_t stream)
+__global__ void __launch_bounds__(64 * kPipedWaves) int8PipedQueue(WorkQueue q, const Int8Args *__restrict__ clips)
 {
-    static bool configured = false;
-    if (!configured) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(int8Piped<NVT, NJB>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipGetLastError();
-        configured = true;
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int item[2];
+    for (int round = 0;; round++) {
+        if (threadIdx.x == 0) item[round & 1] = q.next(blockIdx.x);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(item[round & 1]);
+        if (it < 0) break;
+        int c, k, fBegin, fEnd;
+        q.locate(it, c, k, fBegin, fEnd);
+        int8PipedWindow<NVT, NJB>(clips[c], k, fBegin, fEnd, lds);
     }
-    hipLaunchKernelGGL((int8Piped<NVT, NJB>), dim3(blocks), dim3(kWave * kPipedWaves), shmem, stream, a);
 }
 
-// kind: 0 the one-frame kernel with full bands, 1 the same with the edge row and column waves,
-// 2 pipelined
+// the kernels of one tile shape: kind 0 the one-frame kernel with full bands, 1 the same with the edge
+// row and column waves, 2 pipelined; direct (one block per window and run of frames) or queue
 template <int NVT, int NJB>
-void launchOne(const Int8Args &a, int kind, int blocks, size_t shmem, hipStream_t stream)
+const void *kernelOf(int kind, bool queue)
 {
     switch (kind) {
-    case 0: launchSimple<NVT, NJB, 0>(a, blocks, shmem, stream); break;
-    case 1: launchSimple<NVT, NJB, 2>(a, blocks, shmem, stream); break;
-    default: launchPiped<NVT, NJB>(a, blocks, shmem, stream); break;
+    case 0: return queue ? reinterpret_cast<const void *>(int8Queue<NVT, NJB, 0>)
+                         : reinterpret_cast<const void *>(int8Correlate<NVT, NJB, 0>);
+    case 1: return queue ? reinterpret_cast<const void *>(int8Queue<NVT, NJB, 2>)
+                         : reinterpret_cast<const void *>(int8Correlate<NVT, NJB, 2>);
+    default: return queue ? reinterpret_cast<const void *>(int8PipedQueue<NVT, NJB>)
+                          : reinterpret_cast<const void *>(int8Piped<NVT, NJB>);
     }
 }
 
 template <int NJB>
-bool dispatchNvt(int nvt, const Int8Args &a, int kind, int blocks, size_t shmem, hipStream_t stream)
+const void *kernelNvt(int nvt, int kind, bool queue)
 {
     switch (nvt) {
-    case 1: launchOne<1, NJB>(a, kind, blocks, shmem, stream); return true;
-    case 2: launchOne<2, NJB>(a, kind, blocks, shmem, stream); return true;
-    case 3: launchOne<3, NJB>(a, kind, blocks, shmem, stream); return true;
-    case 4: launchOne<4, NJB>(a, kind, blocks, shmem, stream); return true;
-    case 5: launchOne<5, NJB>(a, kind, blocks, shmem, stream); return true;
-    case 6: launchOne<6, NJB>(a, kind, blocks, shmem, stream); return true;
-    case 7: launchOne<7, NJB>(a, kind, blocks, shmem, stream); return true;
-    case 8: launchOne<8, NJB>(a, kind, blocks, shmem, stream); return true;
-    case 9: launchOne<9, NJB>(a, kind, blocks, shmem, stream); return true;
-    default: return false;
+    case 1: return kernelOf<1, NJB>(kind, queue);
+    case 2: return kernelOf<2, NJB>(kind, queue);
+    case 3: return kernelOf<3, NJB>(kind, queue);
+    case 4: return kernelOf<4, NJB>(kind, queue);
+    case 5: return kernelOf<5, NJB>(kind, queue);
+    case 6: return kernelOf<6, NJB>(kind, queue);
+    case 7: return kernelOf<7, NJB>(kind, queue);
+    case 8: return kernelOf<8, NJB>(kind, queue);
+    case 9: return kernelOf<9, NJB>(kind, queue);
+    default: return nullptr;
     }
 }
 
@@ -671,15 +698,43 @@ Int8Layout int8Layout(const Geometry &g)
     return l;
 }
 
-void launchInt8(Int8Args a, const Int8Layout &l, Int8Variant variant, hipStream_t stream)
+Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue)
 {
-    if (a.fg.n <= 0) return;
     if (!l.supported) throw std::invalid_argument("framecor: the int8 form supports windows up to 129 pixels and search ranges up to 71 pixels");
+    const bool piped = variant == Int8Variant::Piped && l.pipedOk;
+    const bool edge = !piped && variant != Int8Variant::Padded && l.edge;
+    const int kind = piped ? 2 : edge ? 1 : 0, nvt = edge ? l.nvt - 1 : l.nvt;
+    Int8Kernel k{};
+    switch (l.njb) {
+    case 2: k.func = kernelNvt<2>(nvt, kind, queue); break;
+    case 3: k.func = kernelNvt<3>(nvt, kind, queue); break;
+    case 5: k.func = kernelNvt<5>(nvt, kind, queue); break;
+    case 9: k.func = kernelNvt<9>(nvt, kind, queue); break;
+    default: break;
+    }
+    if (!k.func) throw std::invalid_argument("framecor: no int8 kernel for this window and search range");
+    k.threads = kWave * (piped ? kPipedWaves : nvt + (edge ? 2 : 0));
+    k.shmem = piped ? l.pipedShmem : l.shmem;
+    k.variant = piped ? Int8Variant::Piped : edge ? Int8Variant::Simple : Int8Variant::Padded;
+    allowLargeLds(k.func);
+    return k;
+}
+
+void fillInt8Args(Int8Args &a, const Int8Layout &l)
+{
     a.nut = l.nut;
     a.ni4 = l.ni4;
     a.tstride = l.tstride;
     a.cstride = l.cstride;
     a.crows = l.crows;
+    a.alias = l.alias ? 1 : 0;
+}
+
+void launchInt8(Int8Args a, const Int8Layout &l, Int8Variant variant, hipStream_t stream)
+{
+    if (a.fg.n <= 0) return;
+    const Int8Kernel k = int8Kernel(l, variant, false);
+    fillInt8Args(a, l);
     // frames per block: enough blocks to fill the GPU several times over
     const int target = 2048;
     int nfb = (target + a.g.nwin - 1) / a.g.nwin;
@@ -688,21 +743,16 @@ void launchInt8(Int8Args a, const Int8Layout &l, Int8Variant variant, hipStream_
     a.fpb = (a.fg.n + nfb - 1) / nfb;
     a.nfb = (a.fg.n + a.fpb - 1) / a.fpb;
     const int blocks = a.g.nwin * a.nfb;
-    const bool piped = variant == Int8Variant::Piped && l.pipedOk;
-    const bool edge = !piped && variant != Int8Variant::Padded && l.edge;
-    const int kind = piped ? 2 : edge ? 1 : 0, nvt = edge ? l.nvt - 1 : l.nvt;
-    a.alias = l.alias ? 1 : 0;
-    const size_t shmem = piped ? l.pipedShmem : l.shmem;
-    bool ok = false;
-    switch (l.njb) {
-    case 2: ok = dispatchNvt<2>(nvt, a, kind, blocks, shmem, stream); break;
-    case 3: ok = dispatchNvt<3>(nvt, a, kind, blocks, shmem, stream); break;
-    case 5: ok = dispatchNvt<5>(nvt, a, kind, blocks, shmem, stream); break;
-    case 9: ok = dispatchNvt<9>(nvt, a, kind, blocks, shmem, stream); break;
-    default: break;
-    }
-    if (!ok) throw std::invalid_argument("framecor: no int8 kernel for this window and search range");
-    FC_HIP_CHECK(hipGetLastError());
+    void *args[] = {&a};
+    FC_HIP_CHECK(hipLaunchKernel(k.func, dim3(blocks), dim3(k.threads), args, k.shmem, stream));
+}
+
+void launchInt8Queue(const Int8Kernel &k, const WorkQueue &q, const Int8Args *clips, int blocks, hipStream_t stream)
+{
+    WorkQueue qa = q;
+    const Int8Args *c = clips;
+    void *args[] = {&qa, &c};
+    FC_HIP_CHECK(hipLaunchKernel(k.func, dim3(blocks), dim3(k.threads), args, k.shmem, stream));
 }
 
 } // namespace framecor

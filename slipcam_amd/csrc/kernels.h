@@ -9,10 +9,11 @@ namespace framecor {
 
 // the (h + 1) x (w + 1) int64 tables of n frames (n x h x w uint8), pixels and squares
 void launchSummedArea(const uint8_t *frames, int n, int h, int w, int64_t *sat, int64_t *sat2, hipStream_t stream);
-// the W x W box sums of n frames from their tables, n x (h - W + 1) x (w - W + 1) int32 each, and
-// (screen not null) {box, 1 / sqrt(W^2 box2 - box^2) as float bits (0 where that is not positive)}
-void launchBoxSums(const int64_t *sat, const int64_t *sat2, int n, int h, int w, int W, int32_t *box,
-                   int32_t *box2, int2 *screen, hipStream_t stream);
+// the W x W box sums of n frames (n x h x w uint8), n x (h - W + 1) x (w - W + 1) int32 each, straight
+// from the pixels: box (may be null) and box2, the sums of the pixels and of their squares, and (screen
+// not null) {box, 1 / sqrt(W^2 box2 - box^2) as float bits (0 where that is not positive)}
+void launchBoxSums(const uint8_t *frames, int n, int h, int w, int W, int32_t *box, int32_t *box2, int2 *screen,
+                   hipStream_t stream);
 
 // ---- the int8-MFMA form (int8_corr.hip)
 
@@ -48,6 +49,21 @@ enum class Int8Variant { Auto = 0, Simple = 1, Piped = 2, Padded = 3 };
 Int8Layout int8Layout(const Geometry &g);
 void launchInt8(Int8Args a, const Int8Layout &l, Int8Variant variant, hipStream_t stream);
 
+// one int8 kernel: its entry, block size, dynamic LDS and the variant it is (Auto resolved)
+struct Int8Kernel {
+    const void *func;
+    int threads;
+    size_t shmem;
+    Int8Variant variant;
+};
+
+// the kernel of a tile shape (l) and variant; queue: the persistent work-queue form
+Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue);
+// the layout fields of the arguments (nut, ni4, strides, alias)
+void fillInt8Args(Int8Args &a, const Int8Layout &l);
+// the persistent form over a work queue; clips: device array of the clips' arguments (layout filled)
+void launchInt8Queue(const Int8Kernel &k, const WorkQueue &q, const Int8Args *clips, int blocks, hipStream_t stream);
+
 // ---- the FP32 FFT form (fft_corr.hip)
 
 constexpr int kMaxFactors = 12;
@@ -79,6 +95,16 @@ size_t fftShmem(int L);
 void launchFftReference(const FftArgs &a, hipStream_t stream);
 // correlate a group of frames
 void launchFftCorrelate(FftArgs a, hipStream_t stream);
+
+// the persistent work-queue kernel of the FFT form (search-window size L: its LDS)
+struct FftKernel {
+    const void *func;
+    int threads;
+    size_t shmem;
+};
+FftKernel fftQueueKernel(int L);
+// clips: device array of the clips' arguments (np, H, rs, plan, spectra, twiddle filled)
+void launchFftQueue(const FftKernel &k, const WorkQueue &q, const FftArgs *clips, int blocks, hipStream_t stream);
 
 // ---- the differentiable form (diff.hip)
 

@@ -7,12 +7,15 @@
 //               w * 65025 < 2^31 for w < 33000), written as int64 row prefixes;
 //   satCols  -- one thread per table column: the running column sums of the row prefixes, loads
 //               batched eight rows ahead;
-//   boxSums  -- the W x W box sums of the pixels and of their squares at every window position,
-//               (h - W + 1) x (w - W + 1) int32 per frame, from four table reads each (exact: a box of
-//               squares sums to at most W^2 * 65025 < 2^31 for W <= 181), and optionally a packed map of
+//   boxMaps  -- the W x W box sums of the pixels and of their squares at every window position,
+//               (h - W + 1) x (w - W + 1) int32 per frame, straight from the frames with running
+//               vertical sums and prefix scans across columns (no table: the correlator never
+//               materialises the int64 tables; exact: a box of squares sums to at most
+//               W^2 * 65025 < 2^31 for W <= 181), and optionally a packed map of
 //               {sum F, 1 / sqrt(W^2 sum F^2 - (sum F)^2) as FP32 (0 for a flat box)} that every window
 //               touching that position shares (one 8-byte load per surface point in the int8 form's
-//               screen). The correlation kernels read these maps instead of eight int64 table entries.
+//               screen). The correlation kernels read these maps instead of eight int64 table entries;
+//               the maps equal the four-term differences of the tables (cxx/surface.h:21-25) exactly.
 
 #include "common.h"
 #include "kernels.h"
@@ -103,26 +106,81 @@ satCols(int h, int w, int64_t satSize, int64_t *sat, int64_t *sat2)
     }
 }
 
-__global__ void __launch_bounds__(256)
-boxSums(const int64_t *sat, const int64_t *sat2, int h, int w, int W, int64_t satSize, int64_t boxSize,
-        int32_t *box, int32_t *box2, int2 *screen)
+// The W x W box sums straight from the frames (no table): one block of kBoxThreads threads holds
+// kBoxThreads consecutive input columns x0 .. (one per thread) and kBoxThreads - W + 1 output columns;
+// it walks kBoxRows output rows down the frame. Per row each thread moves the vertical W-row sums of
+// its column one row down (one pixel in, one out), the block scans them into prefix sums across the
+// columns, and each output is the difference of two prefixes W apart. The arithmetic is modulo 2^32
+// (uint32): the prefixes of squares may wrap, but every box sum is below 2^31, so the differences are
+// exact. Two LDS buffers (by row parity) let one row's prefixes be read while the next row's are made.
+constexpr int kBoxThreads = 512, kBoxRows = 64;
+
+__global__ void __launch_bounds__(kBoxThreads)
+boxMaps(const uint8_t *frames, int h, int w, int W, int64_t boxSize, int32_t *box, int32_t *box2, int2 *screen)
 {
-    const int bw = w - W + 1;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    const int f = blockIdx.z;
-    if (x >= bw) return;
-    const int64_t stride = w + 1;
-    const int64_t *s = sat + f * satSize, *s2 = sat2 + f * satSize;
-    const int64_t t0 = y * stride + x, t1 = (y + W) * stride + x;
-    const int64_t b = s[t1 + W] - s[t0 + W] - s[t1] + s[t0];
-    const int64_t b2 = s2[t1 + W] - s2[t0 + W] - s2[t1] + s2[t0];
-    const int64_t o = f * boxSize + int64_t(y) * bw + x;
-    box[o] = int32_t(b);
-    box2[o] = int32_t(b2);
-    if (screen) {
-        const int64_t v = int64_t(W) * W * b2 - b * b;
-        screen[o] = make_int2(int32_t(b), __float_as_int(v > 0 ? float(1.0 / sqrt(double(v))) : 0.0f));
+    __shared__ uint32_t pre[2][2][kBoxThreads];
+    __shared__ uint32_t tot[2][2][kBoxThreads / kWave];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
+    const int bh = h - W + 1, bw = w - W + 1, tx = kBoxThreads - W + 1;
+    const int x0 = blockIdx.x * tx, y0 = blockIdx.y * kBoxRows, f = blockIdx.z;
+    const int x = x0 + tid, yEnd = min(y0 + kBoxRows, bh);
+    const bool mine = x < w;
+    const uint8_t *F = frames + size_t(f) * h * w + x;
+    uint32_t vs = 0, vs2 = 0;
+    if (mine)
+        for (int i = 0; i < W; i++) {
+            const uint32_t p = F[size_t(y0 + i) * w];
+            vs += p;
+            vs2 += p * p;
+        }
+    const int64_t n = int64_t(W) * W;
+    // the pixels entering and leaving the vertical sums at the next row
+    uint32_t pin = 0, pout = 0;
+    if (mine && y0 + 1 < yEnd) {
+        pin = F[size_t(y0 + W) * w];
+        pout = F[size_t(y0) * w];
+    }
+    for (int y = y0; y < yEnd; y++) {
+        uint32_t s = vs, s2 = vs2;
+        if (y + 1 < yEnd) {
+            vs += pin - pout;
+            vs2 += pin * pin - pout * pout;
+            if (mine && y + 2 < yEnd) {
+                pin = F[size_t(y + 1 + W) * w];
+                pout = F[size_t(y + 1) * w];
+            }
+        }
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t t = __shfl_up(s, o, kWave), t2 = __shfl_up(s2, o, kWave);
+            if (lane >= o) {
+                s += t;
+                s2 += t2;
+            }
+        }
+        const int b = y & 1;
+        if (lane == kWave - 1) {
+            tot[b][0][wave] = s;
+            tot[b][1][wave] = s2;
+        }
+        __syncthreads();
+        for (int j = 0; j < wave; j++) {
+            s += tot[b][0][j];
+            s2 += tot[b][1][j];
+        }
+        pre[b][0][tid] = s;
+        pre[b][1][tid] = s2;
+        __syncthreads();
+        if (tid < tx && x < bw) {
+            const uint32_t lo = tid ? pre[b][0][tid - 1] : 0u, lo2 = tid ? pre[b][1][tid - 1] : 0u;
+            const int32_t bs = int32_t(pre[b][0][tid + W - 1] - lo), bs2 = int32_t(pre[b][1][tid + W - 1] - lo2);
+            const int64_t o = f * boxSize + int64_t(y) * bw + x;
+            if (box) box[o] = bs;
+            box2[o] = bs2;
+            if (screen) {
+                const int64_t v = n * bs2 - int64_t(bs) * bs;
+                screen[o] = make_int2(bs, __float_as_int(v > 0 ? float(1.0 / sqrt(double(v))) : 0.0f));
+            }
+        }
     }
 }
 
@@ -139,15 +197,15 @@ void launchSummedArea(const uint8_t *frames, int n, int h, int w, int64_t *sat, 
     FC_HIP_CHECK(hipGetLastError());
 }
 
-void launchBoxSums(const int64_t *sat, const int64_t *sat2, int n, int h, int w, int W, int32_t *box,
-                   int32_t *box2, int2 *screen, hipStream_t stream)
+void launchBoxSums(const uint8_t *frames, int n, int h, int w, int W, int32_t *box, int32_t *box2, int2 *screen,
+                   hipStream_t stream)
 {
     if (n <= 0) return;
-    const int64_t satSize = int64_t(h + 1) * (w + 1);
-    const int bh = h - W + 1, bw = w - W + 1;
+    const int bh = h - W + 1, bw = w - W + 1, tx = kBoxThreads - W + 1;
+    if (bh <= 0 || bw <= 0 || W > 181) throw std::invalid_argument("framecor: box sums need 1 <= W <= 181 and W <= h, w");
     const int64_t boxSize = int64_t(bh) * bw;
-    hipLaunchKernelGGL(boxSums, dim3((bw + 255) / 256, bh, n), dim3(256), 0, stream, sat, sat2, h, w, W,
-                       satSize, boxSize, box, box2, screen);
+    hipLaunchKernelGGL(boxMaps, dim3((bw + tx - 1) / tx, (bh + kBoxRows - 1) / kBoxRows, n), dim3(kBoxThreads), 0,
+                       stream, frames, h, w, W, boxSize, box, box2, screen);
     FC_HIP_CHECK(hipGetLastError());
 }
 

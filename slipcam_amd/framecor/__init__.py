@@ -72,6 +72,8 @@ def _load():
     lib.framecor_get_timing.argtypes = [P, P, P]
     lib.framecor_summed_area_device.restype = I
     lib.framecor_summed_area_device.argtypes = [P, I, I, I, P, P, P]
+    lib.framecor_box_sums_device.restype = I
+    lib.framecor_box_sums_device.argtypes = [P, I, I, I, I, P, P, P]
     return lib
 
 
@@ -214,12 +216,12 @@ class Correlator:
         _check(_lib.framecor_set_timing(self._c, int(bool(enable))))
 
     def timing(self):
-        """The GPU milliseconds of the stages of the last correlate call: a dict with 'summed_area',
-        'box_sums', 'correlate' and 'launches' (waits for that call to finish)"""
-        ms = (ctypes.c_float * 3)()
+        """The GPU milliseconds of the stages of the last correlate call: a dict with 'box_sums',
+        'correlate' and 'launches' (waits for that call to finish)"""
+        ms = (ctypes.c_float * 2)()
         launches = ctypes.c_int()
         _check(_lib.framecor_get_timing(self._c, ms, ctypes.byref(launches)))
-        return dict(summed_area=ms[0], box_sums=ms[1], correlate=ms[2], launches=launches.value)
+        return dict(box_sums=ms[0], correlate=ms[1], launches=launches.value)
 
     def _gross(self, gross, n, module):
         ny, nx = self._shape

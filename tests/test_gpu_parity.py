@@ -53,6 +53,28 @@ def test_summed_area_bit_exact(gpu, n, h, w):
     assert np.array_equal(sat2.cpu().numpy(), ref2)
 
 
+@pytest.mark.parametrize('n,h,w,W', [(2, 5, 7, 1), (3, 40, 33, 33), (2, 256, 320, 32), (3, 720, 1280, 64),
+                                     (1, 300, 1100, 181), (2, 1080, 1920, 128), (1, 181, 600, 181)])
+def test_box_sums_bit_exact(gpu, n, h, w, W):
+    """The W x W box sums straight from the pixels equal the four-term differences of the int64
+    summed-area tables (cxx/surface.h:21-25), bit for bit; all-255 rows reach the largest sums"""
+    torch = _torch()
+    rng = np.random.default_rng(n * h + w + W)
+    frames = rng.integers(0, 256, size=(n, h, w)).astype(np.uint8)
+    frames[0, : min(h, W + 3)] = 255
+    d = torch.from_numpy(frames).cuda()
+    bh, bw = h - W + 1, w - W + 1
+    box = torch.empty((n, bh, bw), dtype=torch.int32, device='cuda')
+    box2 = torch.empty_like(box)
+    assert framecor._lib.framecor_box_sums_device(d.data_ptr(), n, h, w, W, box.data_ptr(), box2.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    for p, got in ((1, box), (2, box2)):
+        sat = np.zeros((n, h + 1, w + 1), np.int64)
+        sat[:, 1:, 1:] = (frames.astype(np.int64) ** p).cumsum(1).cumsum(2)
+        ref = sat[:, W:, W:] - sat[:, :-W, W:] - sat[:, W:, :-W] + sat[:, :-W, :-W]
+        assert np.array_equal(got.cpu().numpy().astype(np.int64), ref), p
+
+
 # ---- helpers
 
 def run(method, h, w, down, across, W, R, ref, frames, gross=None):
