@@ -5,14 +5,15 @@ Benchmark of the framecor correlator on one MI355X.
 Workload: the README shape of the reference (720 x 1280 uint8 frames, window 64, search +-48, skip 16:
 2450 windows): 64 synthetic frames of a smoothed-noise texture with a 30-degree fault, camera shake, a
 40 px slip ramp and two fixed text boxes (slipcam_amd.synthetic.fault_video), resident in HBM before
-the timed region. A step is one correlate() of the 64 frames (summed-area tables, box sums and the
-correlation kernel with its fused peak epilogue), device-resident in and out.
+the timed region. A step is one correlate() of the 64 frames (the box-sum maps and the correlation
+kernel with its fused peak epilogue), device-resident in and out.
 
 Both forms are timed: int8 (exact int8-MFMA correlation; the headline value) and fft (the reference's
 FP32 FFT algorithm). The line also carries the per-stage GPU times from hipEvents on the stream of the
 calls, the roofline of the int8 correlation kernel, set_reference times, the PCIe-inclusive rate of the
 host API, and a CPU baseline: the oracle's restatement of the reference CPU algorithm (FP32 FFT,
-OpenMP over windows) on a bounded sample of the same frames.
+OpenMP over windows) on a bounded sample of the same frames. The 'batch' object times the batching
+form: 16 uneven clips in one hipGraph launch per step (framecor.Batch, tuned on its first call).
 
   python bench.py [--steps K] [--warmup W] [--frames 64] [--no-cpu-baseline]
 """
@@ -75,6 +76,56 @@ def time_method(torch, framecor, d, dev, method, steps, warmup):
                 frames_per_launch=n * steps / max(stages['launches'], 1))
 
 
+def time_batch(torch, framecor, specs, vids, method, steps, warmup):
+    """One graph launch over every clip per step (the batching form), tuned on its first call"""
+    batch = framecor.Batch(specs, method=method, tune=True)
+    for c, (ref, _) in enumerate(vids):
+        batch.set_reference(c, ref)
+    frames = [fr for _, fr in vids]
+    t0 = time.perf_counter()
+    batch.correlate(frames)
+    torch.cuda.synchronize()
+    first_ms = (time.perf_counter() - t0) * 1e3
+    for _ in range(warmup):
+        batch.correlate(frames)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.correlate(frames)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    wf = sum(len(s['down']) * len(s['across']) * s['frames'] for s in specs)
+    ops = sum(2.0 * s['window'] ** 2 * (2 * s['search'] + 1) ** 2 * len(s['down']) * len(s['across']) * s['frames']
+              for s in specs)
+    plans = {}
+    for c, s in enumerate(specs):
+        p = batch.plan(c)
+        plans[f'{s["window"]}/{s["search"]}'] = f'{p["variant"]} x{p["frames_per_item"]} {p["blocks_per_cu"]}/CU'
+    info = batch.info()
+    return dict(clips=len(specs), window_frames=wf, ms_per_call=round(ms, 3), windows_per_s=round(wf / ms * 1e3, 1),
+                first_call_ms=round(first_ms, 1), graph_nodes=info['graph_nodes'], stages=info['stages'],
+                tile_shapes=info['shapes'], plans=plans,
+                **({'tops': round(ops / ms * 1e3 / 1e12, 1)} if method == 'int8' else {}))
+
+
+def batch_workload(torch, framecor, synthetic, steps, warmup):
+    """The batching workload: 16 seeded clips of uneven sizes (resolutions 480x640 / 720x1280 /
+    1080x1920, windows 32/64/128, search 16/32/64, 8-256 frames, skip = window / 4), frames made on the
+    GPU; int8 over all clips, fft over those whose search window fits its LDS (L <= 192)"""
+    specs = synthetic.clip_specs()
+    vids = [synthetic.clip_video(s, 'cuda')[:2] for s in specs]
+    torch.cuda.synchronize()
+    out = {'workload': '16 clips: ' + ', '.join(f'{s["height"]}p W{s["window"]} R{s["search"]} x{s["frames"]}'
+                                                for s in specs)}
+    log('bench: batch int8')
+    out['int8'] = time_batch(torch, framecor, specs, vids, 'int8', steps, warmup)
+    keep = [i for i, s in enumerate(specs) if s['window'] + 2 * s['search'] <= 192]
+    log('bench: batch fft')
+    out['fft'] = time_batch(torch, framecor, [specs[i] for i in keep], [vids[i] for i in keep], 'fft', steps, warmup)
+    out['fft']['clips_note'] = f'{len(keep)} of {len(specs)} clips (search window <= 192)'
+    return out
+
+
 def cpu_baseline(d, seconds):
     import oracle
     from slipcam_amd import framecor
@@ -103,6 +154,8 @@ def main():
     p.add_argument('--frames', type=int, default=64)
     p.add_argument('--cpu-seconds', type=float, default=15.0)
     p.add_argument('--no-cpu-baseline', action='store_true')
+    p.add_argument('--no-batch', action='store_true', help='skip the batching workload (16 uneven clips)')
+    p.add_argument('--batch-only', action='store_true', help='time only the batching workload')
     p.add_argument('--methods', default='int8,fft', help='the forms to time (the first is the headline)')
     p.add_argument('--copy-bytes', type=int, default=0,
                    help='instead: time a plain device copy of this many bytes (calibrates the HBM counters)')
@@ -131,6 +184,9 @@ def main():
         print(json.dumps({'copy_bytes': args.copy_bytes, 'ms': dt * 1e3, 'GB/s': 2 * args.copy_bytes / dt / 1e9}))
         return
 
+    if args.batch_only:
+        print(json.dumps({'batch': batch_workload(torch, framecor, synthetic, args.steps, args.warmup)}), flush=True)
+        return
     methods = args.methods.split(',')
     log('bench: generating the workload')
     d = synthetic.fault_video(n=args.frames)
@@ -191,6 +247,8 @@ def main():
                      'kernel': 'int8Correlate', 'kernel_ms_per_launch': round(i8['kernel_ms_per_launch'], 4),
                      'ops_per_launch': ops_launch},
     }
+    if not args.no_batch:
+        line['batch'] = batch_workload(torch, framecor, synthetic, max(2, args.steps // 2), 1)
     if not args.no_cpu_baseline:
         log('bench: CPU baseline')
         line['cpu_baseline'] = cpu_baseline(d, args.cpu_seconds)

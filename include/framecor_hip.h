@@ -136,6 +136,64 @@ int framecor_get_timing(framecor_correlator *c, float *ms, int *launches);
 int framecor_summed_area_device(const uint8_t *frames, int n, int height, int width, int64_t *sat,
                                 int64_t *sat2, void *stream);
 
+/* ---- batches: many clips of uneven sizes in one hipGraph launch (no counterpart in the reference,
+ * whose correlator takes one clip; each clip here is what one reference Correlator(...) would be:
+ * cxx/bindings.cpp:26-45, with a fixed number of frames per call) */
+
+typedef struct framecor_batch framecor_batch;
+
+typedef struct framecor_clip {
+    int height, width;          /* frame size */
+    const int *down;            /* first pixels of the rows of windows (ndown) */
+    int ndown;
+    const int *across;          /* first pixels of the columns of windows (nacross) */
+    int nacross;
+    int window, search;         /* window size and half search range */
+    int stat_size;              /* the SNR window (>= 3; the reference's default is 21) */
+    int frames;                 /* frames of this clip per call */
+} framecor_clip;
+
+/* a batch of nclips clips correlated with one method on one GPU; each clip's parameters are checked
+ * as framecor_create's (FRAMECOR_EINVAL, the message naming the clip). Clips of the same tile shape
+ * (int8: window and search range; fft: window + 2 search) share one persistent work-queue kernel. */
+int framecor_batch_create(const framecor_clip *clips, int nclips, int method, int device, framecor_batch **out);
+
+/* frees the batch and its device memory (NULL is ignored) */
+void framecor_batch_destroy(framecor_batch *b);
+
+/* sets the reference frame of clip (its height x width uint8, host memory); as framecor_set_reference */
+int framecor_batch_set_reference(framecor_batch *b, int clip, const uint8_t *frame);
+
+/* enable 1: the next correlate call first times the candidate plans of every tile shape that has none
+ * (on a sample of its frames) and keeps the fastest; 0 (the default): shapes without a plan use the
+ * automatic one. Every plan gives the same offsets, peaks and covariances (the SNR to rounding). */
+int framecor_batch_set_tuning(framecor_batch *b, int enable);
+
+/* the plan of clip's tile shape: plan[0] the int8 kernel variant (as framecor_set_variant; resolved:
+ * 1 edge waves, 2 pipelined, 3 full bands), plan[1] frames per work item (0: automatic), plan[2]
+ * resident blocks per CU (0: as many as fit), plan[3] 1 if tuned or set */
+int framecor_batch_plan(const framecor_batch *b, int clip, int *plan);
+
+/* sets the plan of clip's tile shape (plan[0..2] as above) */
+int framecor_batch_set_plan(framecor_batch *b, int clip, const int *plan);
+
+/* at most frames (>= 1) frames of each clip per stage (the default: what 2 GiB of box-sum maps per clip
+ * hold, at most 256); a call runs the stages one after the other, each its box sums then its kernels */
+int framecor_batch_set_stage_frames(framecor_batch *b, int frames);
+
+/* info[0] tile shapes, info[1] stages of frames, info[2] nodes of the captured graph (0 before the first
+ * call), info[3] work queues per call */
+int framecor_batch_info(const framecor_batch *b, int *info);
+
+/* correlates every clip: frames[c] (frames_c x height_c x width_c uint8), gross[c] (frames_c x nwin_c x 2
+ * int32, or NULL; gross itself may be NULL) and the outputs offset[c], peak[c], snr[c], cov[c] of clip c
+ * as framecor_correlate's, all in device memory. One graph launch enqueued on stream (NULL: the null
+ * stream), not synchronized; the graph is captured on the first call and again when a pointer changes.
+ * FRAMECOR_ERUNTIME before every clip has a reference. Not re-entrant. */
+int framecor_batch_correlate_device(framecor_batch *b, const uint8_t *const *frames, const int32_t *const *gross,
+                                    float *const *offset, float *const *peak, float *const *snr, float *const *cov,
+                                    void *stream);
+
 /* the W x W box sums of n frames (device memory) at every window position: box and box2,
  * n x (height - window + 1) x (width - window + 1) int32, the sums of the pixels and of their squares;
  * what boxSum (cxx/surface.h:21-25) reads from the summed-area tables, computed here straight from

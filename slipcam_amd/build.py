@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 BUILD = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'libframecor_hip.so')
-SOURCES = ['capi.hip', 'sat.hip', 'int8_corr.hip', 'fft_corr.hip', 'diff.hip']
+SOURCES = ['capi.hip', 'batch.hip', 'sat.hip', 'int8_corr.hip', 'fft_corr.hip', 'diff.hip']
 VERSION = '0.1.0'
 
 

@@ -19,7 +19,7 @@ import os
 
 import numpy as np
 
-__all__ = ['Correlator', 'has_gpu', 'window_grid', '__version__']
+__all__ = ['Batch', 'Correlator', 'has_gpu', 'window_grid', '__version__']
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 _LIB_PATH = os.path.join(_PKG, 'libframecor_hip.so')
@@ -74,6 +74,24 @@ def _load():
     lib.framecor_summed_area_device.argtypes = [P, I, I, I, P, P, P]
     lib.framecor_box_sums_device.restype = I
     lib.framecor_box_sums_device.argtypes = [P, I, I, I, I, P, P, P]
+    lib.framecor_batch_create.restype = I
+    lib.framecor_batch_create.argtypes = [P, I, I, I, ctypes.POINTER(P)]
+    lib.framecor_batch_destroy.restype = None
+    lib.framecor_batch_destroy.argtypes = [P]
+    lib.framecor_batch_set_reference.restype = I
+    lib.framecor_batch_set_reference.argtypes = [P, I, P]
+    lib.framecor_batch_set_tuning.restype = I
+    lib.framecor_batch_set_tuning.argtypes = [P, I]
+    lib.framecor_batch_plan.restype = I
+    lib.framecor_batch_plan.argtypes = [P, I, P]
+    lib.framecor_batch_set_plan.restype = I
+    lib.framecor_batch_set_plan.argtypes = [P, I, P]
+    lib.framecor_batch_set_stage_frames.restype = I
+    lib.framecor_batch_set_stage_frames.argtypes = [P, I]
+    lib.framecor_batch_info.restype = I
+    lib.framecor_batch_info.argtypes = [P, P]
+    lib.framecor_batch_correlate_device.restype = I
+    lib.framecor_batch_correlate_device.argtypes = [P, P, P, P, P, P, P, P]
     return lib
 
 
@@ -286,6 +304,166 @@ class Correlator:
                 self._c, frames.data_ptr(), n, None if g is None else g.data_ptr(), out['offsets'].data_ptr(),
                 out['peak'].data_ptr(), out['snr'].data_ptr(), out['cov'].data_ptr(), stream))
         return out
+
+
+class _Clip(ctypes.Structure):
+    _fields_ = [('height', ctypes.c_int), ('width', ctypes.c_int), ('down', ctypes.c_void_p), ('ndown', ctypes.c_int),
+                ('across', ctypes.c_void_p), ('nacross', ctypes.c_int), ('window', ctypes.c_int),
+                ('search', ctypes.c_int), ('stat_size', ctypes.c_int), ('frames', ctypes.c_int)]
+
+
+_VARIANTS = {'auto': 0, 'simple': 1, 'piped': 2, 'padded': 3}
+
+
+class Batch:
+    """
+    Many clips of uneven sizes correlated in one hipGraph launch (no counterpart in the reference: each
+    clip is what one Correlator would be, with a fixed number of frames per call)
+
+    clips: a sequence of dicts with height, width, down, across, window, search, frames (per call) and
+    optionally stat_size (21). method: 'int8' or 'fft'. tune: time the candidate plans of every tile
+    shape on the first call and keep the fastest (otherwise the automatic plan).
+
+      batch = framecor.Batch(clips, method='int8')
+      for c, ref in enumerate(refs):
+          batch.set_reference(c, ref)
+      results = batch.correlate(frames)      # frames[c]: frames x height x width uint8 per clip
+    """
+
+    def __init__(self, clips, *, method='int8', device=0, tune=True):
+        self._b = None
+        m = _METHODS.get(method)
+        if m is None:
+            raise ValueError(f"framecor: the method must be fft or int8, not {method}")
+        self._keep = []
+        arr = (_Clip * len(clips))()
+        self._clips = []
+        for i, c in enumerate(clips):
+            down = np.ascontiguousarray(c['down'], np.int32)
+            across = np.ascontiguousarray(c['across'], np.int32)
+            self._keep += [down, across]
+            arr[i] = _Clip(int(c['height']), int(c['width']), down.ctypes.data, len(down), across.ctypes.data,
+                           len(across), int(c['window']), int(c['search']), int(c.get('stat_size', 21)),
+                           int(c['frames']))
+            self._clips.append(dict(height=int(c['height']), width=int(c['width']), shape=(len(down), len(across)),
+                                    frames=int(c['frames'])))
+        handle = ctypes.c_void_p()
+        _check(_lib.framecor_batch_create(ctypes.cast(arr, ctypes.c_void_p), len(clips), m, int(device),
+                                          ctypes.byref(handle)))
+        self._b = handle
+        self._destroy = _lib.framecor_batch_destroy
+        self._device = int(device)
+        self._method = method
+        _check(_lib.framecor_batch_set_tuning(self._b, int(bool(tune))))
+
+    def __del__(self):
+        b, destroy = getattr(self, '_b', None), getattr(self, '_destroy', None)
+        if b and destroy is not None:
+            destroy(b)
+            self._b = None
+
+    def __len__(self):
+        return len(self._clips)
+
+    @property
+    def method(self):
+        return self._method
+
+    def set_reference(self, clip, frame):
+        """Set the reference frame of one clip (its height x width, uint8)"""
+        c = self._clips[clip]
+        if _is_device_tensor(frame):
+            frame = frame.cpu().numpy()
+        frame = np.ascontiguousarray(frame, np.uint8)
+        if frame.shape != (c['height'], c['width']):
+            raise ValueError(f'framecor: the reference frame of clip {clip} must be height x width')
+        _check(_lib.framecor_batch_set_reference(self._b, int(clip), _ptr(frame)))
+
+    def plan(self, clip):
+        """The plan of the clip's tile shape: kernel variant, frames per work item (0: automatic),
+        resident blocks per CU (0: as many as fit), and whether it was tuned or set"""
+        p = (ctypes.c_int * 4)()
+        _check(_lib.framecor_batch_plan(self._b, int(clip), p))
+        names = {v: k for k, v in _VARIANTS.items()}
+        return dict(variant=names[p[0]], frames_per_item=p[1], blocks_per_cu=p[2], tuned=bool(p[3]))
+
+    def set_plan(self, clip, variant='auto', frames_per_item=0, blocks_per_cu=0):
+        """Set the plan of the clip's tile shape (every clip of that shape)"""
+        p = (ctypes.c_int * 3)(_VARIANTS[variant], int(frames_per_item), int(blocks_per_cu))
+        _check(_lib.framecor_batch_set_plan(self._b, int(clip), p))
+
+    def set_stage_frames(self, frames):
+        """At most this many frames of each clip per stage (default: what 2 GiB of box-sum maps hold)"""
+        _check(_lib.framecor_batch_set_stage_frames(self._b, int(frames)))
+
+    def info(self):
+        """tile shapes, stages of frames, nodes of the captured graph, work queues per call"""
+        p = (ctypes.c_int * 4)()
+        _check(_lib.framecor_batch_info(self._b, p))
+        return dict(shapes=p[0], stages=p[1], graph_nodes=p[2], queues=p[3])
+
+    def _staging(self, key, c, shape, dtype):
+        """A device buffer of the batch kept across calls (stable pointers: the captured graph is reused)"""
+        import torch
+        buf = self._bufs.get((key, c))
+        if buf is None or tuple(buf.shape) != tuple(shape) or buf.dtype != dtype:
+            buf = torch.empty(shape, dtype=dtype, device=torch.device('cuda', self._device))
+            self._bufs[(key, c)] = buf
+        return buf
+
+    def correlate(self, frames, gross=None):
+        """
+        Correlate every clip: frames[c] is clip c's frames x height x width uint8 (torch tensors on the
+        GPU, or numpy arrays), gross[c] None, frames x 2 or frames x down x across x 2. Returns one dict
+        per clip as Correlator.correlate, computed by one graph launch on the current stream: numpy arrays
+        for host inputs; for device inputs, torch tensors on the GPU that belong to the batch and are
+        overwritten by its next call (clone them to keep them). The graph is captured on the first call
+        and again whenever the frames or gross tensors given are other tensors than last time.
+        """
+        import torch
+        if not hasattr(self, '_bufs'):
+            self._bufs = {}
+        if len(frames) != len(self._clips):
+            raise ValueError(f'framecor: {len(self._clips)} clips, {len(frames)} frame arrays')
+        host = not all(_is_device_tensor(f) for f in frames)
+        dev = torch.device('cuda', self._device)
+        outs, fr, gr = [], [], []
+        for c, (info, f) in enumerate(zip(self._clips, frames)):
+            n, h, w = info['frames'], info['height'], info['width']
+            ny, nx = info['shape']
+            if not _is_device_tensor(f):
+                f = np.ascontiguousarray(f, np.uint8)
+                if f.shape != (n, h, w):
+                    raise ValueError(f'framecor: the frames of clip {c} must be {n} x {h} x {w} uint8')
+                f = self._staging('frames', c, (n, h, w), torch.uint8).copy_(torch.from_numpy(f))
+            if f.dtype != torch.uint8 or tuple(f.shape) != (n, h, w) or f.device != dev:
+                raise ValueError(f'framecor: the frames of clip {c} must be {n} x {h} x {w} uint8 on {dev}')
+            if not f.is_contiguous():
+                f = self._staging('frames', c, (n, h, w), torch.uint8).copy_(f)
+            g = None if gross is None else gross[c]
+            if g is not None:
+                g = torch.as_tensor(np.asarray(g) if not isinstance(g, torch.Tensor) else g)
+                if tuple(g.shape) == (n, 2):
+                    g = g[:, None, None, :].expand(n, ny, nx, 2)
+                elif tuple(g.shape) != (n, ny, nx, 2):
+                    raise ValueError(f'framecor: gross of clip {c} must be frames x 2 or frames x down x across x 2')
+                g = self._staging('gross', c, (n, ny, nx, 2), torch.int32).copy_(g)
+            outs.append({k: self._staging(k, c, s, torch.float32)
+                         for k, s in (('offsets', (n, ny, nx, 2)), ('peak', (n, ny, nx)), ('snr', (n, ny, nx)),
+                                      ('cov', (n, ny, nx, 3)))})
+            fr.append(f.data_ptr())
+            gr.append(0 if g is None else g.data_ptr())
+        P = ctypes.c_void_p * len(self._clips)
+        arrays = [P(*fr), P(*gr) if gross is not None else None] + [P(*[o[k].data_ptr() for o in outs])
+                                                                      for k in ('offsets', 'peak', 'snr', 'cov')]
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        _check(_lib.framecor_batch_correlate_device(self._b, *[None if a is None else ctypes.cast(a, ctypes.c_void_p)
+                                                               for a in arrays], stream))
+        self._inflight = frames   # the caller's tensors stay alive while the launch may read them
+        if host:
+            torch.cuda.current_stream(dev).synchronize()
+            return [{k: v.cpu().numpy() for k, v in o.items()} for o in outs]
+        return outs
 
 
 class _NumpyOps:

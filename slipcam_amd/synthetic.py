@@ -99,6 +99,60 @@ def fault_video(n=64, height=720, width=1280, angle=30.0, slip=40.0, centre=30.0
                 window=64, search=48, skip=16)
 
 
+RESOLUTIONS = ((480, 640), (720, 1280), (1080, 1920))
+
+
+def clip_specs(n=16, seed=3, frames=(8, 256), resolutions=RESOLUTIONS, windows=(32, 64, 128), searches=(16, 32, 64)):
+    """The batching workload: n clips, each with a resolution, window and search range drawn from the
+    given sets, a frame count uniform in frames[0]..frames[1], skip = window / 4 and a shift sigma
+    uniform in 1..5 px (a list of dicts; the grid from the reference's window_grid rule)"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        h, w = resolutions[rng.integers(len(resolutions))]
+        W = int(windows[rng.integers(len(windows))])
+        R = int(searches[rng.integers(len(searches))])
+        nf = int(rng.integers(frames[0], frames[1] + 1))
+        sigma = float(rng.uniform(1.0, 5.0))
+        skip = W // 4
+        down = R + skip * np.arange((h - 2 * R - W) // skip)
+        across = R + skip * np.arange((w - 2 * R - W) // skip)
+        out.append(dict(height=h, width=w, window=W, search=R, skip=skip, frames=nf, sigma=sigma, down=down,
+                        across=across, seed=seed * 1000 + i))
+    return out
+
+
+def clip_video(spec, device):
+    """The reference (numpy uint8) and frames (torch uint8 on device) of one clip: the texture of its
+    resolution moved by a per-frame shift N(0, sigma^2) per axis (clipped to +-(search - 1) px),
+    bilinearly (borders reflected), plus noise sigma 2 DN, rounded and clipped; and the shifts (n x 2)"""
+    import torch
+    h, w, n = spec['height'], spec['width'], spec['frames']
+    base = texture(h, w)
+    ref = to_uint8(base)
+    rng = np.random.default_rng(spec['seed'])
+    lim = spec['search'] - 1
+    shifts = np.clip(rng.normal(0.0, spec['sigma'], size=(n, 2)), -lim, lim)
+    pad = int(np.ceil(np.abs(shifts).max())) + 2 if n else 2
+    img = torch.tensor(np.pad(base, pad, mode='reflect'), dtype=torch.float32, device=device)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(spec['seed'])
+    frames = torch.empty((n, h, w), dtype=torch.uint8, device=device)
+    for f in range(n):
+        # out(y, x) = base(y - dy, x - dx): the padded image at (pad - dy + y, pad - dx + x)
+        sy, sx = pad - shifts[f, 0], pad - shifts[f, 1]
+        iy, ix = int(np.floor(sy)), int(np.floor(sx))
+        fy, fx = float(sy - iy), float(sx - ix)
+        a = img[iy:iy + h, ix:ix + w]
+        b = img[iy:iy + h, ix + 1:ix + 1 + w]
+        c = img[iy + 1:iy + 1 + h, ix:ix + w]
+        d = img[iy + 1:iy + 1 + h, ix + 1:ix + 1 + w]
+        moved = (1 - fy) * ((1 - fx) * a + fx * b) + fy * ((1 - fx) * c + fx * d)
+        noise = torch.randn((h, w), generator=gen, device=device) * 2.0
+        frames[f] = torch.clamp(torch.round(moved + noise), 0, 255).to(torch.uint8)
+    return ref, frames, shifts
+
+
 def gross_offsets(n, ny, nx, low=-20, high=20, seed=3):
     """Per-window integer gross offsets, n x ny x nx x 2 int32, uniform in low..high"""
     return np.random.default_rng(seed).integers(low, high + 1, size=(n, ny, nx, 2)).astype(np.int32)

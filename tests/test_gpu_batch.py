@@ -1,0 +1,149 @@
+"""
+The batching form (csrc/batch.hip, framecor.Batch): clips of uneven sizes in one hipGraph launch of
+persistent work-queue kernels. Per clip, the batch's outputs are bit-identical to the clip run alone
+through the Correlator with the same kernel variant (a size-independent property: the batch only
+moves the same per-window, per-frame computation between blocks), for both forms, with and without
+gross offsets, over several stages of frames, over more clips than one queue launch holds, after
+tuning, and on replays of the captured graph.
+"""
+
+import numpy as np
+import pytest
+
+from slipcam_amd import framecor, synthetic
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip('torch')
+
+KEYS = ('offsets', 'peak', 'snr', 'cov')
+
+
+def every_shape(frames=(1, 4), seed=11, resolutions=synthetic.RESOLUTIONS):
+    """One clip per (window, search) pair of the batching workload, resolutions drawn at random"""
+    specs = []
+    i = 0
+    for W in (32, 64, 128):
+        for R in (16, 32, 64):
+            s = synthetic.clip_specs(n=1, seed=seed + i, frames=frames, resolutions=resolutions,
+                                     windows=(W,), searches=(R,))[0]
+            specs.append(s)
+            i += 1
+    return specs
+
+
+def videos(specs):
+    out = []
+    for s in specs:
+        ref, fr, _ = synthetic.clip_video(s, 'cuda')
+        out.append((ref, fr))
+    return out
+
+
+def solo(spec, ref, frames, method, variant='auto', gross=None):
+    cor = framecor.Correlator(spec['height'], spec['width'], spec['down'], spec['across'], window=spec['window'],
+                              search=spec['search'], method=method)
+    if method == 'int8':
+        cor.set_variant(variant)
+    cor.set_reference(ref)
+    r = cor.correlate(frames, gross=gross)
+    return {k: (v.cpu().numpy() if isinstance(v, torch.Tensor) else v) for k, v in r.items()}
+
+
+def same(a, b, where=''):
+    for k in KEYS:
+        x = a[k].cpu().numpy() if isinstance(a[k], torch.Tensor) else a[k]
+        assert np.array_equal(x, b[k], equal_nan=True), (where, k)
+
+
+def run_batch(specs, vids, method, tune=False, gross=None, stage_frames=None):
+    batch = framecor.Batch(specs, method=method, tune=tune)
+    for c, (ref, _) in enumerate(vids):
+        batch.set_reference(c, ref)
+    if stage_frames:
+        batch.set_stage_frames(stage_frames)
+    out = batch.correlate([fr for _, fr in vids], gross=gross)
+    torch.cuda.synchronize()
+    return batch, out
+
+
+def test_int8_batch_matches_each_clip_alone(gpu):
+    specs = every_shape()
+    vids = videos(specs)
+    batch, out = run_batch(specs, vids, 'int8')
+    assert batch.info()['shapes'] == 9 and batch.info()['graph_nodes'] > 0
+    for c, (s, (ref, fr)) in enumerate(zip(specs, vids)):
+        expect = solo(s, ref, fr, 'int8', batch.plan(c)['variant'])
+        same(out[c], expect, f'clip {c} W {s["window"]} R {s["search"]} {s["height"]}x{s["width"]}')
+        assert np.isfinite(expect['offsets']).mean() > 0.5
+
+
+def test_fft_batch_matches_each_clip_alone(gpu):
+    specs = [s for s in every_shape(seed=31) if s['window'] + 2 * s['search'] <= 192]
+    vids = videos(specs)
+    batch, out = run_batch(specs, vids, 'fft')
+    for c, (s, (ref, fr)) in enumerate(zip(specs, vids)):
+        same(out[c], solo(s, ref, fr, 'fft'), f'clip {c}')
+
+
+def test_tuned_batch_replays_the_same(gpu):
+    """Tuning picks a plan per tile shape; the results are those of each clip alone with the chosen
+    variant, and a replay of the captured graph gives the same bits"""
+    specs = every_shape(frames=(2, 6), seed=51, resolutions=((480, 640), (720, 1280)))
+    vids = videos(specs)
+    batch, out = run_batch(specs, vids, 'int8', tune=True)
+    first = [{k: v.clone() for k, v in o.items()} for o in out]
+    nodes = batch.info()['graph_nodes']
+    for c, (s, (ref, fr)) in enumerate(zip(specs, vids)):
+        plan = batch.plan(c)
+        assert plan['tuned'] and plan['frames_per_item'] >= 1
+        same(out[c], solo(s, ref, fr, 'int8', plan['variant']), f'clip {c} plan {plan}')
+    again = batch.correlate([fr for _, fr in vids])
+    torch.cuda.synchronize()
+    assert batch.info()['graph_nodes'] == nodes
+    for c in range(len(specs)):
+        same(again[c], {k: v.cpu().numpy() for k, v in first[c].items()}, f'replay clip {c}')
+
+
+@pytest.mark.parametrize('method', ['int8', 'fft'])
+def test_batch_with_gross_offsets(gpu, method):
+    specs = synthetic.clip_specs(n=4, seed=71, frames=(2, 5), windows=(32, 64), searches=(16, 32))
+    vids = videos(specs)
+    gross = []
+    for c, s in enumerate(specs):
+        n, ny, nx = s['frames'], len(s['down']), len(s['across'])
+        g = synthetic.gross_offsets(n, ny, nx, seed=c) if c % 2 else np.random.default_rng(c).integers(-20, 21, (n, 2))
+        gross.append(g.astype(np.int32))
+    _, out = run_batch(specs, vids, method, gross=gross)
+    for c, (s, (ref, fr)) in enumerate(zip(specs, vids)):
+        same(out[c], solo(s, ref, fr, method, gross=gross[c]), f'clip {c}')
+
+
+def test_many_clips_over_stages(gpu):
+    """More clips of one shape than a queue launch holds (48), frames over three stages"""
+    specs = []
+    for i in range(50):
+        h, w = 96 + 8 * (i % 3), 128 + 16 * (i % 4)
+        down, across = framecor.window_grid(h, w, 16, 8, 12)
+        specs.append(dict(height=h, width=w, window=16, search=8, down=down, across=across,
+                          frames=5 + i % 3, sigma=1.0 + (i % 5) * 0.5, seed=900 + i))
+    vids = videos(specs)
+    batch, out = run_batch(specs, vids, 'int8', stage_frames=2)
+    info = batch.info()
+    # stages 0-2: all 50 clips (two queue launches each); stage 3: the 16 clips of 7 frames (one)
+    assert info['stages'] == 4 and info['queues'] == 7
+    for c, (s, (ref, fr)) in enumerate(zip(specs, vids)):
+        same(out[c], solo(s, ref, fr, 'int8', batch.plan(c)['variant']), f'clip {c}')
+
+
+def test_host_arrays(gpu):
+    specs = synthetic.clip_specs(n=3, seed=81, frames=(1, 3), resolutions=((480, 640),))
+    vids = videos(specs)
+    batch = framecor.Batch(specs, method='int8', tune=False)
+    for c, (ref, _) in enumerate(vids):
+        batch.set_reference(c, ref)
+    dev = batch.correlate([fr for _, fr in vids])
+    dev = [{k: v.cpu().numpy() for k, v in o.items()} for o in dev]
+    host = batch.correlate([fr.cpu().numpy() for _, fr in vids])
+    for c in range(len(specs)):
+        same(host[c], dev[c], f'clip {c}')

@@ -102,6 +102,23 @@ def test_no_gpu_is_a_runtime_error():
         framecor.Correlator(240, 320, [0], [0], window=32, search=8)
 
 
+def test_batch_arguments_without_gpu():
+    """A batch checks every clip as a Correlator would, naming the clip, before any GPU call"""
+    good = dict(height=240, width=320, down=[16, 48], across=[16, 80], window=32, search=8, frames=4)
+    with pytest.raises(ValueError, match=r'window must be at least 3 pixels \(clip 1\)'):
+        framecor.Batch([good, dict(good, window=2)], method='int8')
+    with pytest.raises(ValueError, match=r'outside the frame \(across 400\) \(clip 2\)'):
+        framecor.Batch([good, good, dict(good, across=[400])], method='int8')
+    with pytest.raises(ValueError, match=r'fft method .* \(clip 0\)'):
+        framecor.Batch([dict(good, height=1000, width=1000, window=128, search=64)], method='fft')
+    with pytest.raises(ValueError, match='frames must not be negative'):
+        framecor.Batch([dict(good, frames=-1)], method='int8')
+    with pytest.raises(ValueError, match='method'):
+        framecor.Batch([good], method='fp16')
+    with pytest.raises(ValueError, match='at least one clip'):
+        framecor.Batch([], method='int8')
+
+
 def test_product_never_uses_the_oracle():
     """The package neither imports nor loads anything from oracle/ (the checker)"""
     for path in glob.glob(os.path.join(ROOT, 'slipcam_amd', '**', '*.*'), recursive=True):
