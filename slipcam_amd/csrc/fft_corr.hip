@@ -92,7 +92,8 @@ __global__ void __launch_bounds__(kFftThreads) fftReference(FftArgs a)
 }
 
 // window k against frames fBegin .. fEnd - 1, by the whole block
-__device__ inline void fftWindow(const FftArgs &a, int k, int fBegin, int fEnd, unsigned char *lds)
+__device__ __attribute__((always_inline)) inline void fftWindow(const FftArgs &a, int k, int fBegin, int fEnd,
+                                                                unsigned char *lds)
 {
     const FftShared s = carve(lds, a);
     const Geometry &g = a.g;

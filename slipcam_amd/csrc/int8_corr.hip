@@ -76,6 +76,27 @@ __device__ inline float screenNcc(int raw, int fs, float rsig, const Moments &m)
     return float(numerator(raw, fs, m)) * rsig * m.rst;
 }
 
+// a lane's two largest screened values, the index (u M + v) and raw value of the largest; equal values:
+// the lower index
+struct Top2 {
+    float f1, f2;
+    int i1, r1;
+};
+
+__device__ inline Top2 top2Init() { return Top2{-3.0f, -3.0f, 0x7fffffff, 0}; }
+
+__device__ inline void top2Add(Top2 &t, float x, int idx, int raw)
+{
+    if (x > t.f1 || (x == t.f1 && idx < t.i1)) {
+        t.f2 = t.f1;
+        t.f1 = x;
+        t.i1 = idx;
+        t.r1 = raw;
+    }
+    else if (x > t.f2)
+        t.f2 = x;
+}
+
 // the raw int32 surface of one 16-row band (rows 16 band .. + 15, all NVT v tiles) of a staged chip
 template <int NVT, int NJB>
 __device__ inline void bandMfma(v4i (&acc)[NVT], const int8_t *chip, const int8_t *tmpl, int band, int lane, int ni4,
@@ -153,40 +174,32 @@ __device__ inline v4i edgeRowMfma(const int8_t *chip, const int8_t *tmpl, int c,
 // (0..c-1) on M and the 16 rows of a block on N:
 //   A[m][(i', jb, kk)] = F[16 m + i'][16 c + 16 jb + kk],  B[(i', jb, kk)][n] = T[i' - n][16 jb + kk],
 //   C[m][n] = sum over i' in [0, W + 15), jb in [0, ceil(W / 16))  -> r[16 m + n][16 c]
-// (i', jb) pairs are taken four at a time, one per 16-lane group. Rows m >= c of
-// C are discarded (their lanes read the chip at m = c - 1); a B fragment outside the template (i' - n
-// outside 0..W-1, or a pair past the last) reads bytes 0..15 of template row 0, which are zero.
-__device__ inline v4i edgeColMfma(const int8_t *chip, const int8_t *tmpl, int c, int W, int lane, int tstride,
-                                  int cstride)
+// i' is taken four at a time (i' = 4 q + g4, one per 16-lane group), jb inner (immediate LDS offsets).
+// Rows m >= c of C are discarded (their lanes read the chip at m = c - 1); a B fragment outside the
+// template (i' - n outside 0..W-1) reads the zero block (16 NJB zero bytes).
+template <int NJB>
+__device__ inline v4i edgeColMfma(const int8_t *chip, const int8_t *tmpl, const int8_t *zero, int c, int W, int lane,
+                                  int tstride, int cstride)
 {
     v4i acc = v4i{0, 0, 0, 0};
     const int g4 = lane >> 4, m16 = lane & 15;
-    const int nJ = (W + 15) >> 4, groups = (W + 15) * nJ, steps = (groups + 3) >> 2;
-    const unsigned mJ = unsigned((0x100000000ull + nJ - 1) / unsigned(nJ));
+    const int nJ = (W + 15) >> 4, nq = (W + 18) >> 2;
     const int8_t *abase = chip + 16 * min(m16, c - 1) * cstride + 16 * c;
-    int t = 0;
-    for (; t + 4 <= steps; t += 4) {
-        v4i av[4], bv[4];
+    for (int q = 0; q < nq; q++) {
+        const int ip = 4 * q + g4, ti = ip - m16;
+        const bool inside = ip < W + 15 && ti >= 0 && ti < W;
+        const int8_t *arow = abase + (ip < W + 15 ? ip : 0) * cstride;
+        const int8_t *brow = inside ? tmpl + ti * tstride + 16 : zero;
+        v4i av[NJB], bv[NJB];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int s = 4 * (t + q) + g4;
-            const int ip = nJ == 1 ? s : int(__umulhi(unsigned(s), mJ)), jb = s - ip * nJ;
-            const int ti = ip - m16;
-            const bool inside = s < groups && ti >= 0 && ti < W;
-            av[q] = *reinterpret_cast<const v4i *>(abase + (s < groups ? ip * cstride + 16 * jb : 0));
-            bv[q] = *reinterpret_cast<const v4i *>(tmpl + (inside ? ti * tstride + 16 + 16 * jb : 0));
-        }
+        for (int jb = 0; jb < NJB; jb++)
+            if (jb < nJ) {
+                av[jb] = *reinterpret_cast<const v4i *>(arow + 16 * jb);
+                bv[jb] = *reinterpret_cast<const v4i *>(brow + 16 * jb);
+            }
 #pragma unroll
-        for (int q = 0; q < 4; q++) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[q], bv[q], acc, 0, 0, 0);
-    }
-    for (; t < steps; t++) {
-        const int s = 4 * t + g4;
-        const int ip = nJ == 1 ? s : int(__umulhi(unsigned(s), mJ)), jb = s - ip * nJ;
-        const int ti = ip - m16;
-        const bool inside = s < groups && ti >= 0 && ti < W;
-        const v4i av = *reinterpret_cast<const v4i *>(abase + (s < groups ? ip * cstride + 16 * jb : 0));
-        const v4i bv = *reinterpret_cast<const v4i *>(tmpl + (inside ? ti * tstride + 16 + 16 * jb : 0));
-        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc, 0, 0, 0);
+        for (int jb = 0; jb < NJB; jb++)
+            if (jb < nJ) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[jb], bv[jb], acc, 0, 0, 0);
     }
     return acc;
 }
@@ -195,16 +208,27 @@ __device__ inline v4i edgeColMfma(const int8_t *chip, const int8_t *tmpl, int c,
 // t0 .. t0 + nt - 1 of the block (t: this thread's index among them)
 __device__ inline void loadChip(int8_t *chip, const uint8_t *src, int w, int L, int cstride, int t, int nt)
 {
+    // thread t takes the 16-byte column chunk t % cpr of rows t / cpr, t / cpr + nt / cpr, ... (cpr <= nt)
     const int cpr = L >> 4;
-    for (int c = t; c < L * cpr; c += nt) {
-        const int r = c / cpr, cc = c - r * cpr;
-        uint4 v;
-        __builtin_memcpy(&v, src + size_t(r) * w + 16 * cc, 16);
-        v.x ^= 0x80808080u;
-        v.y ^= 0x80808080u;
-        v.z ^= 0x80808080u;
-        v.w ^= 0x80808080u;
-        *reinterpret_cast<uint4 *>(chip + r * cstride + 16 * cc) = v;
+    if (cpr > 0) {
+        const int rowsPer = nt / cpr, r0 = t / cpr, cc = t - r0 * cpr;
+        if (r0 < rowsPer) {
+            const uint8_t *s = src + size_t(r0) * w + 16 * cc;
+            int8_t *d = chip + r0 * cstride + 16 * cc;
+            const size_t sstep = size_t(rowsPer) * w;
+            const int dstep = rowsPer * cstride;
+            for (int r = r0; r < L; r += rowsPer) {
+                uint4 v;
+                __builtin_memcpy(&v, s, 16);
+                v.x ^= 0x80808080u;
+                v.y ^= 0x80808080u;
+                v.z ^= 0x80808080u;
+                v.w ^= 0x80808080u;
+                *reinterpret_cast<uint4 *>(d) = v;
+                s += sstep;
+                d += dstep;
+            }
+        }
     }
     const int tail = L & 15;
     if (tail)
@@ -325,46 +349,59 @@ __device__ inline void helperScreen(const Int8Args &a, const Moments &mom, const
     }
 }
 
-// one wave: the peak of frame f from the helpers' results, then the SNR over the stat box and the 3 x 3
-// FP64 fit and covariance on exact values (cxx/surface.h:71-146 formulas)
-__device__ inline void helperFinish(const Int8Args &a, const Moments &mom, const int *S, const FrameWindow &fw, int f,
-                                    const WaveBest *part, int nh, int lane)
+// The SNR box around the peak (ub, vb) of an M x M surface: n = min(stat, M) points a side, from
+// (peak - n / 2), clipped to the surface (cxx/surface.h:108-124)
+struct SnrBox {
+    int ub, vb, ulo, uhi, vlo, vhi, bwid, items;
+    bool inside;   // the peak has its 3 x 3 neighbours
+};
+
+__device__ inline SnrBox snrBox(int ib, int M, int stat)
+{
+    SnrBox b;
+    b.ub = ib / M;
+    b.vb = ib - b.ub * M;
+    const int nn = stat < M ? stat : M;
+    b.ulo = max(b.ub - nn / 2, 0);
+    b.uhi = min(b.ub - nn / 2 + nn, M);
+    b.vlo = max(b.vb - nn / 2, 0);
+    b.vhi = min(b.vb - nn / 2 + nn, M);
+    b.bwid = b.vhi - b.vlo;
+    b.items = (b.uhi - b.ulo) * b.bwid;
+    b.inside = b.ub > 0 && b.ub < M - 1 && b.vb > 0 && b.vb < M - 1;
+    return b;
+}
+
+// one wave: the outputs of frame f from its exact peak (best, ib): the SNR over the stat box and the 3 x 3
+// FP64 fit and covariance on exact values (cxx/surface.h:71-146 formulas). The raw surface S is the whole
+// M x M surface, or (boxed) only the SNR box, row-major from its corner.
+__device__ inline void finishPeak(const Int8Args &a, const Moments &mom, const int *S, bool boxed, const FrameWindow &fw,
+                                  int f, double best, int ib, int lane)
 {
     const Geometry &g = a.g;
     const int M = g.M;
-    if (!fw.valid) {
-        if (lane == 0) invalidateOut(fw.q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
-        return;
-    }
-    double best = -3.0;
-    int ib = 0x7fffffff;
-    for (int t = 0; t < nh; t++) bestOf(best, ib, part[t].value, part[t].index);
-    if (!(best > 0)) {
+    if (!fw.valid || !(best > 0)) {
         if (lane == 0) invalidateOut(fw.q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
         return;
     }
     const int64_t boff = f * g.boxSize + int64_t(fw.sy) * g.bw + fw.sx;
     const int2 *scr = a.fg.screen + boff;
     const int32_t *box2 = a.fg.box2 + boff;
-    const int ub = ib / M, vb = ib - ub * M;
-    const int nn = g.stat < M ? g.stat : M;
-    const int ulo = max(ub - nn / 2, 0), uhi = min(ub - nn / 2 + nn, M);
-    const int vlo = max(vb - nn / 2, 0), vhi = min(vb - nn / 2 + nn, M);
-    const int bwid = vhi - vlo, items = (uhi - ulo) * bwid;
-    const bool inside = ub > 0 && ub < M - 1 && vb > 0 && vb < M - 1;
+    const SnrBox b = snrBox(ib, M, g.stat);
+    const int sStride = boxed ? b.bwid : M, su = boxed ? b.ulo : 0, sv = boxed ? b.vlo : 0;
     double sum = 0.0;
     // the stat box, kBatch points per lane in flight; lanes 0..8 also take the 3 x 3 neighbours
     double mine = 0.0;
-    for (int it0 = lane; it0 < items; it0 += kBatch * kWave) {
+    for (int it0 = lane; it0 < b.items; it0 += kBatch * kWave) {
         int raw[kBatch], fs[kBatch], f2v[kBatch];
         bool ok[kBatch];
 #pragma unroll
         for (int t = 0; t < kBatch; t++) {
             const int it = it0 + t * kWave;
-            ok[t] = it < items;
+            ok[t] = it < b.items;
             if (ok[t]) {
-                const int u = ulo + it / bwid, v = vlo + it % bwid, o = u * g.bw + v;
-                raw[t] = S[u * M + v];
+                const int u = b.ulo + it / b.bwid, v = b.vlo + it % b.bwid, o = u * g.bw + v;
+                raw[t] = S[(u - su) * sStride + v - sv];
                 fs[t] = scr[o].x;
                 f2v[t] = box2[o];
             }
@@ -376,40 +413,57 @@ __device__ inline void helperFinish(const Int8Args &a, const Moments &mom, const
                 sum += d * d;
             }
     }
-    if (inside && lane < 9) {
-        const int u = ub + lane / 3 - 1, v = vb + lane % 3 - 1, o = u * g.bw + v;
-        mine = exactNcc(S[u * M + v], scr[o].x, box2[o], mom);
+    if (b.inside && lane < 9) {
+        const int u = b.ub + lane / 3 - 1, v = b.vb + lane % 3 - 1, o = u * g.bw + v;
+        mine = exactNcc(S[(u - su) * sStride + v - sv], scr[o].x, box2[o], mom);
     }
     sum = waveSumD(sum);
     double fsq[3][3];
     for (int i = 0; i < 9; i++) fsq[i / 3][i % 3] = __shfl(mine, i, kWave);
     fsq[1][1] = best;
     if (lane == 0)
-        writePeakF64(fsq, inside, ub, vb, g.R, fw.gy, fw.gx, sum, items, mom.nd, fw.q, a.fg.offset, a.fg.peak,
+        writePeakF64(fsq, b.inside, b.ub, b.vb, g.R, fw.gy, fw.gx, sum, b.items, mom.nd, fw.q, a.fg.offset, a.fg.peak,
                      a.fg.snr, a.fg.cov);
+}
+
+// one wave: the peak of frame f from the helpers' results over the whole raw surface S
+__device__ inline void helperFinish(const Int8Args &a, const Moments &mom, const int *S, const FrameWindow &fw, int f,
+                                    const WaveBest *part, int nh, int lane)
+{
+    double best = -3.0;
+    int ib = 0x7fffffff;
+    for (int t = 0; t < nh; t++) bestOf(best, ib, part[t].value, part[t].index);
+    finishPeak(a, mom, S, false, fw, f, best, ib, lane);
 }
 
 // Window k against frames fBegin .. fEnd - 1, by the whole block (NVT + EW waves).
 // EW = 0: ceil(M / 16) = NVT full bands; EW = 2 (M = 16 NVT + 1): NVT core bands plus a wave for the edge
-// row and one for the edge column (EW = 1, both in one wave, makes that wave the critical path)
+// row and one for the edge column.
+//
+// Per frame: every wave runs its MFMAs (a band, the edge row or the edge column) into registers and
+// screens its own points there (FP32 screen from the frame's screen map, exact FP64 NCC of the points
+// within the screen's error of the wave's maximum) into parts[wave]; barrier B2; every wave takes the
+// block's exact peak from parts and writes its raw values inside the SNR box around it to LDS, and the
+// chip of the next valid frame is staged (every MFMA of this frame is over); barrier B3; then the
+// finisher wave (the edge-row wave, whose MFMA share is the smallest, or wave 0) writes the outputs
+// while the other waves start the next frame's MFMAs. LDS: the template, a zero block, the chip, the
+// SNR box (min(stat, M)^2 int32) and the waves' results -- no full surface.
 template <int NVT, int NJB, int EW>
-__device__ inline void int8Window(const Int8Args &a, int k, int fBegin, int fEnd, unsigned char *lds)
+__device__ __attribute__((always_inline)) inline void int8Window(const Int8Args &a, int k, int fBegin, int fEnd,
+                                                                 unsigned char *lds)
 {
     const Geometry &g = a.g;
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6, nthreads = blockDim.x;
+    const int tid = threadIdx.x, lane0 = tid & (kWave - 1), wave = tid >> 6, nthreads = blockDim.x;
     const int nw = nthreads / kWave;
     const int W = g.W, M = g.M, L = g.L;
-
-    // the raw surface after the chip, or (a.alias) over it: then the MFMAs of a frame must be over before
-    // the surface is written, and the frame finished before the next chip is staged
-    const int tbytes = 4 * a.ni4 * a.tstride, cbytes = a.crows * a.cstride, sbytes = 4 * ((M * M + 3) & ~3);
+    const Int8Lds sh = int8Lds(a.ni4, a.tstride, a.crows, a.cstride, NJB, M, g.stat, nw);
     int8_t *tmpl = reinterpret_cast<int8_t *>(lds);
-    int8_t *chip = tmpl + tbytes;
-    int *surf = reinterpret_cast<int *>(a.alias ? chip : chip + cbytes);
-    WaveBest *parts = reinterpret_cast<WaveBest *>(chip + (a.alias ? max(cbytes, sbytes) : cbytes + sbytes));
+    const int8_t *zero = tmpl + sh.zero;
+    int8_t *chip = tmpl + sh.chip;
+    int *boxbuf = reinterpret_cast<int *>(lds + sh.box);
+    WaveBest *parts = reinterpret_cast<WaveBest *>(lds + sh.parts);
 
-    loadTemplate(lds, tbytes + cbytes, tmpl, a.tmpl + size_t(k) * W * W, W, a.tstride, tid, nthreads);
-    __syncthreads();
+    loadTemplate(lds, sh.box, tmpl, a.tmpl + size_t(k) * W * W, W, a.tstride, tid, nthreads);
     const float normk = a.wd.norm[k];
     const int y0 = a.wd.down[k / g.nx], x0 = a.wd.across[k % g.nx];
     Moments mom;
@@ -418,53 +472,150 @@ __device__ inline void int8Window(const Int8Args &a, int k, int fBegin, int fEnd
     mom.nd = double(mom.n);
     mom.tau = double(a.wd.tsum[k] - 128 * mom.n);
     mom.rst = mom.tvar > 0 ? float(1.0 / sqrt(double(mom.tvar))) : 0.0f;
-    const int g4 = lane >> 4, m16 = lane & 15;
+    const bool band = EW == 0 || wave < NVT;
+    const int finisher = EW == 0 ? 0 : NVT;
 
-    for (int f = fBegin; f < fEnd; f++) {
-        const FrameWindow fw = frameWindow(a, k, f, y0, x0, normk);
-        if (!fw.valid) {
-            if (tid == 0) invalidateOut(fw.q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
-            continue;   // uniform over the block
-        }
-        // the chip (the previous frame's MFMAs ended before its second barrier)
+    // the first valid frame (the invalid ones before it written as such), its chip
+    int f = fBegin;
+    FrameWindow fw;
+    for (;; f++) {
+        if (f >= fEnd) break;
+        fw = frameWindow(a, k, f, y0, x0, normk);
+        if (fw.valid) break;
+        if (tid == 0) invalidateOut(fw.q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
+    }
+    __syncthreads();   // the template (and the zeroed areas)
+    if (f < fEnd)
         loadChip(chip, a.fg.frames + size_t(f) * g.pixels + size_t(fw.sy) * g.w + fw.sx, g.w, L, a.cstride, tid,
                  nthreads);
-        __syncthreads();   // B1 (also: wave 0 has finished reading the previous surface)
-        if (EW == 0 || wave < NVT) {
-            // a band: rows 16 wave .. + 15, NVT tiles (with an edge: the 16 c x 16 c core)
-            v4i acc[NVT];
+    __syncthreads();   // B1
+
+    while (f < fEnd) {
+        // the lane's coordinates anew each frame: the per-point indices derived from them are recomputed
+        // here (a few VALU) rather than hoisted out of the loop and kept live (spilled) across the MFMAs
+        int lane = lane0;
+        asm volatile("" : "+v"(lane));
+        const int g4 = lane >> 4, m16 = lane & 15;
+        // the MFMAs of this wave's share into registers
+        v4i acc[NVT];
+        v4i es = v4i{0, 0, 0, 0};
+        if (band)
             bandMfma<NVT, NJB>(acc, chip, tmpl, wave, lane, a.ni4, a.tstride, a.cstride);
-            if (a.alias) __syncthreads();
-#pragma unroll
-            for (int v1 = 0; v1 < NVT; v1++)
+        else if (wave == NVT)
+            es = edgeRowMfma<NJB>(chip, tmpl, NVT, lane, a.ni4, a.tstride, a.cstride);
+        else
+            es = edgeColMfma<NJB>(chip, tmpl, zero, NVT, W, lane, a.tstride, a.cstride);
+
+        // this lane's points: fn(raw, u, v)
+        auto visit = [&](auto &&fn) {
+            if (band) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int u = 16 * wave + 4 * g4 + r, v = 16 * v1 + m16;
-                    if (u < M && v < M) surf[u * M + v] = acc[v1][r];
+                    const int u = 16 * wave + 4 * g4 + r;
+#pragma unroll
+                    for (int v1 = 0; v1 < NVT; v1++) {
+                        const int v = 16 * v1 + m16;
+                        if (EW != 0 || (u < M && v < M)) fn(acc[v1][r], u, v);
+                    }
                 }
-        }
-        else {
-            // the edge row u = 16 c and the edge column v = 16 c (c = NVT)
-            const bool doRow = EW == 1 || wave == NVT, doCol = EW == 1 || wave == NVT + 1;
-            v4i rs = v4i{0, 0, 0, 0}, cs = v4i{0, 0, 0, 0};
-            if (doRow) rs = edgeRowMfma<NJB>(chip, tmpl, NVT, lane, a.ni4, a.tstride, a.cstride);
-            if (doCol) cs = edgeColMfma(chip, tmpl, NVT, W, lane, a.tstride, a.cstride);
-            if (a.alias) __syncthreads();
+            }
+            else if (wave == NVT) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int blk = 4 * g4 + r, v = 16 * blk + m16;
+                    if (blk <= NVT && v < M) fn(es[r], 16 * NVT, v);
+                }
+            }
+            else {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int blk = 4 * g4 + r;
+                    if (blk < NVT) fn(es[r], 16 * blk + m16, 16 * NVT);
+                }
+            }
+        };
+
+        // the screen of this wave's points, then the exact best of its candidates
+        const int64_t boff = f * g.boxSize + int64_t(fw.sy) * g.bw + fw.sx;
+        const int2 *scr = a.fg.screen + boff;
+        const int32_t *box2 = a.fg.box2 + boff;
+        Top2 t = top2Init();
+        if (band) {
+            // a row of NVT points per r: one row pointer, the columns at immediate offsets
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int blk = 4 * g4 + r, v = 16 * blk + m16;
-                if (doRow && blk <= NVT && v < M) surf[16 * NVT * M + v] = rs[r];
-                if (doCol && blk < NVT) surf[(16 * blk + m16) * M + 16 * NVT] = cs[r];
+                const int u = 16 * wave + 4 * g4 + r;
+                const int2 *row = scr + u * g.bw + m16;
+                int2 p[NVT];
+#pragma unroll
+                for (int v1 = 0; v1 < NVT; v1++)
+                    p[v1] = (EW != 0 || (u < M && 16 * v1 + m16 < M)) ? row[16 * v1] : make_int2(0, 0);
+#pragma unroll
+                for (int v1 = 0; v1 < NVT; v1++)
+                    if (EW != 0 || (u < M && 16 * v1 + m16 < M))
+                        top2Add(t, screenNcc(acc[v1][r], p[v1].x, __int_as_float(p[v1].y), mom), u * M + 16 * v1 + m16,
+                                acc[v1][r]);
             }
         }
-        __syncthreads();   // B2
-        helperScreen(a, mom, surf, fw, f, wave, nw, lane, parts + wave);
-        __syncthreads();   // B3
-        if (wave == 0) helperFinish(a, mom, surf, fw, f, parts, nw, lane);
-        if (a.alias) __syncthreads();
+        else
+            visit([&](int raw, int u, int v) {
+                const int2 p = scr[u * g.bw + v];
+                top2Add(t, screenNcc(raw, p.x, __int_as_float(p.y), mom), u * M + v, raw);
+            });
+        float fmax = t.f1;
+        int imax = t.i1;
+        waveBest(fmax, imax);
+        const float thr = fmax - 1e-5f * fabsf(fmax) - 1e-30f;
+        double dbest = -3.0;
+        int dib = 0x7fffffff;
+        if (__any(t.f2 >= thr)) {
+            // a lane holds two candidates: every point within the screen's error of the maximum
+            visit([&](int raw, int u, int v) {
+                const int o = u * g.bw + v;
+                const int2 p = scr[o];
+                if (screenNcc(raw, p.x, __int_as_float(p.y), mom) >= thr)
+                    bestOf(dbest, dib, exactNcc(raw, p.x, box2[o], mom), u * M + v);
+            });
+        }
+        else if (t.f1 >= thr) {
+            const int u = t.i1 / M, v = t.i1 - u * M, o = u * g.bw + v;
+            dbest = exactNcc(t.r1, scr[o].x, box2[o], mom);
+            dib = t.i1;
+        }
+        waveBest(dbest, dib);
+        if (lane == 0) {
+            parts[wave].value = dbest;
+            parts[wave].index = dib;
+        }
+        __syncthreads();   // B2: every MFMA of this frame is over; parts are in
+
+        // the block's exact peak; this wave's raw values inside its SNR box
+        double best = -3.0;
+        int ib = 0x7fffffff;
+        for (int w = 0; w < nw; w++) bestOf(best, ib, parts[w].value, parts[w].index);
+        if (best > 0) {
+            const SnrBox b = snrBox(ib, M, g.stat);
+            if (!band || (16 * wave < b.uhi && 16 * wave + 16 > b.ulo))
+                visit([&](int raw, int u, int v) {
+                    if (u >= b.ulo && u < b.uhi && v >= b.vlo && v < b.vhi) boxbuf[(u - b.ulo) * b.bwid + v - b.vlo] = raw;
+                });
+        }
+        // the next valid frame and its chip
+        const FrameWindow cur = fw;
+        const int fc = f;
+        for (f++;; f++) {
+            if (f >= fEnd) break;
+            fw = frameWindow(a, k, f, y0, x0, normk);
+            if (fw.valid) break;
+            if (tid == 0) invalidateOut(fw.q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
+        }
+        if (f < fEnd)
+            loadChip(chip, a.fg.frames + size_t(f) * g.pixels + size_t(fw.sy) * g.w + fw.sx, g.w, L, a.cstride, tid,
+                     nthreads);
+        __syncthreads();   // B3: the SNR box and the next chip are in
+        if (wave == finisher) finishPeak(a, mom, boxbuf, true, cur, fc, best, ib, lane);
     }
 }
-
 template <int NVT, int NJB, int EW>
 __global__ void __launch_bounds__(64 * (NVT + EW))
 __attribute__((amdgpu_waves_per_eu(NVT + EW <= 8 ? 4 : 1))) int8Correlate(Int8Args a)
@@ -511,7 +662,8 @@ __attribute__((amdgpu_waves_per_eu(NVT + EW <= 8 ? 4 : 1))) int8Queue(WorkQueue 
 // the next one, so one frame's MFMAs overlap the staging, screening and peak fitting of its neighbours.
 
 template <int NVT, int NJB>
-__device__ inline void int8PipedWindow(const Int8Args &a, int k, int fBegin, int fEnd, unsigned char *lds)
+__device__ __attribute__((always_inline)) inline void int8PipedWindow(const Int8Args &a, int k, int fBegin, int fEnd,
+                                                                      unsigned char *lds)
 {
     const Geometry &g = a.g;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6, nthreads = blockDim.x;
@@ -682,12 +834,9 @@ Int8Layout int8Layout(const Geometry &g)
     if (cs % 2 == 0) cs++;   // an odd number of 16-byte units: conflict-free 16-byte row reads
     l.cstride = 16 * cs;
     l.crows = 16 * l.nut + 4 * l.ni4;
-    const size_t tb = size_t(4 * l.ni4) * l.tstride, cb = size_t(l.crows) * l.cstride;
-    // (one WaveBest per wave: nut, or nut + 1 with the edge row and column in two waves)
-    const size_t sb = sizeof(int) * size_t((g.M * g.M + 3) & ~3), pb = sizeof(WaveBest) * (l.nut + 1);
-    l.shmem = tb + cb + sb + pb;
-    l.alias = l.shmem > 80 * 1024;   // two blocks per CU need at most 80 KB each; else share chip and surface
-    if (l.alias) l.shmem = tb + (cb > sb ? cb : sb) + pb;
+    // (one result per wave: nut, or nut + 1 with the edge row and column in two waves)
+    l.shmem = size_t(int8Lds(l.ni4, l.tstride, l.crows, l.cstride, l.njb, g.M, g.stat, l.nut + 1).total);
+    l.alias = false;
     l.supported = l.njb > 0 && l.nvt <= 9 && l.shmem <= 160 * 1024;
     l.edge = g.M % 16 == 1 && g.M > 1;
     // the pipelined kernel: two chips, two raw surfaces, two sets of helper results

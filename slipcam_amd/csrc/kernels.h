@@ -31,10 +31,28 @@ struct Int8Args {
     int alias;             // the one-frame kernel: the raw surface shares the chip's LDS
 };
 
+// The LDS of the one-frame-at-a-time int8 kernel (byte offsets): the template rows, a zero block of
+// 16 njb bytes, the chip, the SNR box (min(stat, M)^2 int32) and one result per wave
+struct Int8Lds {
+    int zero, chip, box, parts, total;
+};
+
+__host__ __device__ inline Int8Lds int8Lds(int ni4, int tstride, int crows, int cstride, int njb, int M, int stat, int nw)
+{
+    Int8Lds l;
+    l.zero = 4 * ni4 * tstride;
+    l.chip = l.zero + ((16 * njb + 15) & ~15);
+    l.box = l.chip + ((crows * cstride + 15) & ~15);
+    const int nn = stat < M ? stat : M;
+    l.parts = l.box + ((4 * nn * nn + 15) & ~15);
+    l.total = l.parts + 16 * nw;
+    return l;
+}
+
 struct Int8Layout {
     int njb, nvt, nut, ni4, tstride, cstride, crows;
     size_t shmem;         // the one-frame-at-a-time kernel
-    bool alias;           // its raw surface over its chip (large windows)
+    bool alias;           // (unused: the one-frame kernel keeps no full surface)
     bool supported;
     bool edge;            // M = 16 c + 1: c core bands plus a wave for the edge row and one for the edge column
     size_t pipedShmem;    // the pipelined producer/helper kernel

@@ -102,6 +102,24 @@ def test_no_gpu_is_a_runtime_error():
         framecor.Correlator(240, 320, [0], [0], window=32, search=8)
 
 
+def test_boxes_mask_brute_force():
+    """The windows flagged by the text boxes (track_offsets.py:27-32): a window is masked iff some pixel
+    of it lies inside some box"""
+    from slipcam_amd import track
+    down, across = framecor.window_grid(720, 1280, 64, 48, 16)
+    boxes = [(870, 25, 1240, 68), (40, 650, 630, 700), (0, 0, 1, 1)]
+    got = track.boxes_mask(down, across, 64, boxes)
+    inside = np.zeros((720, 1280), bool)
+    for x0, y0, x1, y1 in boxes:
+        inside[y0:y1, x0:x1] = True
+    cs = np.pad(inside.cumsum(0).cumsum(1), ((1, 0), (1, 0)))
+    for i, y in enumerate(down):
+        for j, x in enumerate(across):
+            hits = cs[y + 64, x + 64] - cs[y, x + 64] - cs[y + 64, x] + cs[y, x]
+            assert got[i, j] == (hits > 0), (y, x)
+    assert 0 < got.sum() < got.size
+
+
 def test_batch_arguments_without_gpu():
     """A batch checks every clip as a Correlator would, naming the clip, before any GPU call"""
     good = dict(height=240, width=320, down=[16, 48], across=[16, 80], window=32, search=8, frames=4)
