@@ -22,7 +22,8 @@ import numpy as np
 __all__ = ['Batch', 'Correlator', 'has_gpu', 'window_grid', '__version__']
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-_LIB_PATH = os.path.join(_PKG, 'libframecor_hip.so')
+# FRAMECOR_LIBRARY: another build of the library (for example one compiled with other options)
+_LIB_PATH = os.environ.get('FRAMECOR_LIBRARY') or os.path.join(_PKG, 'libframecor_hip.so')
 
 _OK, _EINVAL, _ERUNTIME, _ENOMEM = 0, 1, 2, 3
 _METHODS = {'fft': 0, 'int8': 1}
