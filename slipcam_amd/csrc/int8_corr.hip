@@ -85,15 +85,16 @@ struct Top2 {
 
 __device__ inline Top2 top2Init() { return Top2{-3.0f, -3.0f, 0x7fffffff, 0}; }
 
-// branch-free: the value that does not become the largest raises the second (when x wins, the old
-// largest is at least the second)
 __device__ inline void top2Add(Top2 &t, float x, int idx, int raw)
 {
-    const bool win = x > t.f1 || (x == t.f1 && idx < t.i1);
-    t.f2 = fmaxf(t.f2, win ? t.f1 : x);
-    t.f1 = win ? x : t.f1;
-    t.i1 = win ? idx : t.i1;
-    t.r1 = win ? raw : t.r1;
+    if (x > t.f1 || (x == t.f1 && idx < t.i1)) {
+        t.f2 = t.f1;
+        t.f1 = x;
+        t.i1 = idx;
+        t.r1 = raw;
+    }
+    else if (x > t.f2)
+        t.f2 = x;
 }
 
 // the raw int32 surface of one 16-row band (rows 16 band .. + 15, all NVT v tiles) of a staged chip
