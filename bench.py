@@ -31,6 +31,22 @@ METRIC = 'ms/frame + windows/s @720p 2450 win 64² ±48: int8-MFMA vs FP32-FFT'
 INT8_PEAK_TOPS = 5000.0        # MI355X dense int8 MFMA, TOP/s
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 
+# HBM traffic per launch of the two correlation kernels on this workload at 64 frames per launch, in KiB
+# per dispatch as rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE read them (one pass each), and the factors
+# that turn them into bytes: what the same counters read for a plain 256 MiB device copy
+# (profiles/counters_int8Correlate_fftCorrelate_720p_64frames.md holds the passes and the commands)
+TRAFFIC_KIB = {'int8': (1.693e6, 5032.0), 'fft': (1.113e7, 2.083e4)}
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+TRAFFIC_FRAMES = 64
+
+
+def traffic_bytes(method, frames):
+    """HBM bytes per launch from the profiled counters, or None for a workload other than the profiled one."""
+    if frames != TRAFFIC_FRAMES:
+        return None
+    fetch, write = TRAFFIC_KIB[method]
+    return round(1024 * (FETCH_SCALE * fetch + WRITE_SCALE * write))
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -209,6 +225,10 @@ def main():
     L = W + 2 * R
     spectra_bytes = nwin * L * (L // 2 + 1) * 8
     fft_bytes_frame = spectra_bytes + d['frames'][0].nbytes + nwin * 7 * 4
+    # the int8 kernel's unique HBM bytes per launch: the reference windows, and per frame the frame,
+    # its screen map (8 bytes) and sum-of-squares map (4 bytes) per window position, and the outputs
+    bh, bw = d['frames'].shape[1] - W + 1, d['frames'].shape[2] - W + 1
+    i8_bytes_launch = nwin * W * W + i8['frames_per_launch'] * (d['frames'][0].nbytes + 12 * bh * bw + nwin * 7 * 4)
     fft_gbs = fft_bytes_frame * ff['frames_per_launch'] / (ff['kernel_ms_per_launch'] * 1e-3) / 1e9
 
     line = {
@@ -235,7 +255,9 @@ def main():
                 'stage_ms_per_frame': {k: round(v, 5) for k, v in ff['stage_ms_per_frame'].items()},
                 'kernel_ms_per_launch': round(ff['kernel_ms_per_launch'], 4),
                 'roofline': {'bound': 'hbm', 'achieved': round(fft_gbs, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                             'frac': round(fft_gbs / HBM_PEAK_GBS, 4)},
+                             'frac': round(fft_gbs / HBM_PEAK_GBS, 4),
+                             'traffic': traffic_bytes('fft', ff['frames_per_launch']),
+                             'algorithmic_bytes': fft_bytes_frame * ff['frames_per_launch']},
                 'set_reference_ms': round(ff['set_reference_ms'], 1),
                 'host_api_ms_per_frame': round(ff['host_api_ms_per_frame'], 4)},
         'int8_over_fft': round(ff['ms_per_frame'] / i8['ms_per_frame'], 3),
@@ -243,7 +265,9 @@ def main():
         'set_reference_ms': round(i8['set_reference_ms'], 1),
         'host_api_ms_per_frame': round(i8['host_api_ms_per_frame'], 4),
         'roofline': {'bound': 'mfma', 'achieved': round(achieved, 1), 'peak': INT8_PEAK_TOPS, 'unit': 'TOP/s',
-                     'frac': round(achieved / INT8_PEAK_TOPS, 4), 'traffic': None,
+                     'frac': round(achieved / INT8_PEAK_TOPS, 4),
+                     'traffic': traffic_bytes('int8', i8['frames_per_launch']),
+                     'algorithmic_bytes': i8_bytes_launch,
                      'kernel': 'int8Correlate', 'kernel_ms_per_launch': round(i8['kernel_ms_per_launch'], 4),
                      'ops_per_launch': ops_launch},
     }

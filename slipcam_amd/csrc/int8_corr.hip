@@ -616,9 +616,15 @@ __device__ __attribute__((always_inline)) inline void int8Window(const Int8Args 
         if (wave == finisher) finishPeak(a, mom, boxbuf, true, cur, fc, best, ib, lane);
     }
 }
+// The register budget of a block of NVT + EW waves: blocks of more than 8 waves run one per CU (no
+// limit below the 512 registers of a lone wave); a template of 112 or more columns (NJB >= 8) leaves
+// LDS for at most three blocks per CU, so its waves take up to 168 registers (3 per SIMD) rather than
+// spill at 128; the rest keep 4 waves per SIMD (two blocks of 8 waves per CU)
+constexpr int int8WavesPerEu(int nvt, int njb, int ew) { return nvt + ew > 8 ? 1 : njb >= 8 ? 3 : 4; }
+
 template <int NVT, int NJB, int EW>
 __global__ void __launch_bounds__(64 * (NVT + EW))
-__attribute__((amdgpu_waves_per_eu(NVT + EW <= 8 ? 4 : 1))) int8Correlate(Int8Args a)
+__attribute__((amdgpu_waves_per_eu(int8WavesPerEu(NVT, NJB, EW)))) int8Correlate(Int8Args a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     const int logical = xcdRemap(blockIdx.x, gridDim.x);
@@ -632,7 +638,7 @@ __attribute__((amdgpu_waves_per_eu(NVT + EW <= 8 ? 4 : 1))) int8Correlate(Int8Ar
 // included). Each block takes the next item of its XCD's queue, then steals from the other XCDs'.
 template <int NVT, int NJB, int EW>
 __global__ void __launch_bounds__(64 * (NVT + EW))
-__attribute__((amdgpu_waves_per_eu(NVT + EW <= 8 ? 4 : 1))) int8Queue(WorkQueue q, const Int8Args *__restrict__ clips)
+__attribute__((amdgpu_waves_per_eu(int8WavesPerEu(NVT, NJB, EW)))) int8Queue(WorkQueue q, const Int8Args *__restrict__ clips)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int item[2];
