@@ -142,6 +142,28 @@ def batch_workload(torch, framecor, synthetic, steps, warmup):
     return out
 
 
+def time_periodicity(d, repeats=3):
+    """slipcam-auto's periodicity of the reference frame on this geometry (python/slipcam/auto_slip.py:308):
+    the exact NCC surface of every window with its surroundings and its second peak, through the host
+    API (one synchronous call: the frame in, one float64 per window out)"""
+    from slipcam_amd import framecor
+    h, w = d['ref'].shape
+    W, R = d['window'], d['search']
+    down, across = framecor.window_grid(h, w, W, R, d['skip'])
+    cor = framecor.Correlator(h, w, down, across, window=W, search=R, method='int8')
+    cor.set_reference(d['ref'])
+    cor.periodicity(d['ref'])
+    ms = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        per = cor.periodicity(d['ref'])
+        ms.append((time.perf_counter() - t0) * 1e3)
+    M = 2 * R + 1
+    ops = 2.0 * W * W * M * M * len(down) * len(across)
+    return {'ms': round(min(ms), 3), 'windows': int(per.size), 'tops': round(ops / (min(ms) * 1e-3) / 1e12, 1),
+            'periodic_windows': int((per > 0.8).sum()), 'api': 'Correlator.periodicity (slipcam_amd.auto_slip.periodicity)'}
+
+
 def cpu_baseline(d, seconds):
     import oracle
     from slipcam_amd import framecor
@@ -271,6 +293,7 @@ def main():
                      'kernel': 'int8Correlate', 'kernel_ms_per_launch': round(i8['kernel_ms_per_launch'], 4),
                      'ops_per_launch': ops_launch},
     }
+    line['periodicity'] = time_periodicity(d)
     if not args.no_batch:
         line['batch'] = batch_workload(torch, framecor, synthetic, max(2, args.steps // 2), 1)
     if not args.no_cpu_baseline:

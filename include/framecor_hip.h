@@ -202,6 +202,27 @@ int framecor_batch_correlate_device(framecor_batch *b, const uint8_t *const *fra
 int framecor_box_sums_device(const uint8_t *frames, int n, int height, int width, int window, int32_t *box,
                              int32_t *box2, void *stream);
 
+/* The periodicity of every window (python/slipcam/auto_slip.py:105-129 `periodicity`): the NCC of the
+ * window's reference template (set_reference first) with its own search window in frame (h x w uint8,
+ * host; the reference frame for the periodicity), each window at its own position; out[k] (nwin
+ * doubles, host, k = row x nx + column) = the largest value of the surface at a local maximum (equal to
+ * the largest value of its 5 x 5 neighbourhood clipped to the surface, as scipy's maximum_filter with
+ * size 5) outside the rows and columns search - exclude .. search + exclude (bounds as a Python slice
+ * of the surface), 0 where no such point exists, 1 where the template is flat. The NCC is exact (integer
+ * sums, FP64 normalization), which the reference evaluates with FP64 FFTs. Every search window must lie
+ * in the frame (else FRAMECOR_EINVAL). Synchronous. */
+int framecor_periodicity(framecor_correlator *c, const uint8_t *frame, int exclude, double *out);
+
+/* The exact FP64 NCC surfaces of windows k0 .. k0 + count - 1 against one frame (h x w uint8, device),
+ * each window at its own position with no gross offset: surfaces (count x M x M doubles, device)
+ * [i][u][v] = the NCC of window k0 + i with the frame at shift (u - search, v - search), exact integer
+ * sums normalized in FP64 as (W^2 sum FT - sum T sum F) / sqrt((W^2 sum T^2 - (sum T)^2)
+ * (W^2 sum F^2 - (sum F)^2)), 0 where either variance is 0 (cxx/surface.h:30-35). The surface that
+ * cxx/cpu.cpp:138-154 evaluates in FP32 per window before its argmax. Enqueued on stream, not
+ * synchronized; one call at a time per correlator (it owns the frame's box maps). */
+int framecor_ncc_surfaces_device(framecor_correlator *c, const uint8_t *frame, int k0, int count, double *surfaces,
+                                 void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,8 @@ def lib():
     L.oracle_correlate_fft.argtypes = [P, P, I, P, P, P, P, P, P]
     L.oracle_correlate_direct.restype = I
     L.oracle_correlate_direct.argtypes = [P, P, I, P, P, P, P, P, P, P]
+    L.oracle_periodicity.restype = I
+    L.oracle_periodicity.argtypes = [P, P, I, P]
     L.oracle_reference_stats.restype = I
     L.oracle_reference_stats.argtypes = [P, P, P, P]
     L.oracle_reference_spectra.restype = P
@@ -212,6 +214,17 @@ class OracleCorrelator:
                                             _ptr(out['peak']), _ptr(out['snr']), _ptr(out['cov']))
             if rc:
                 raise RuntimeError(f'oracle_correlate_fft: {rc}')
+        return out
+
+    def periodicity(self, frame, exclude=5):
+        """The second peak of each window's exact NCC with its own surroundings in frame (down x across
+        float64), the periodicity of slipcam-auto"""
+        frame = np.ascontiguousarray(frame, np.uint8)
+        assert frame.shape == (self.h, self.w)
+        out = np.empty(self.shape, np.float64)
+        rc = lib().oracle_periodicity(self._c, _ptr(frame), int(exclude), _ptr(out))
+        if rc:
+            raise RuntimeError(f'oracle_periodicity: {rc}')
         return out
 
     def correlate_direct(self, frames, gross=None, windows=None, surfaces=False):

@@ -775,6 +775,98 @@ int oracle_correlate_direct(oracle_cor *c, const uint8_t *frames, int n, const i
     return failed ? -1 : 0;
 }
 
+/* a bound of the Python slice a[i:] or a[:i] over a length-M axis (positive step) */
+static int py_slice_bound(int64_t i, int M)
+{
+    if (i < 0) {
+        i += M;
+        if (i < 0) i = 0;
+    }
+    return i > M ? M : (int)i;
+}
+
+/* The periodicity of python/slipcam/auto_slip.py:105-129: per window, the NCC of the reference window
+ * with the frame around it (its search window at its own position, no gross offset), then the largest
+ * value at a local maximum (a point equal to the largest value of its 5 x 5 neighbourhood clipped to the
+ * surface: scipy.ndimage.maximum_filter, size 5, whose default 'reflect' mode repeats in-surface values)
+ * outside the rows and columns of the slice [search - exclude : search + exclude + 1] (:124-125); 0 if
+ * there is none (:126); 1 where the template is flat, tn < 1e-6 with tn the FP64 norm of the template
+ * minus its FP64 mean (:113-116). The NCC is the exact one (int64 sums, FP64 from exact moments, as
+ * ncc_f64); the reference evaluates the same quotient with FP64 FFTs (:117-122).
+ * out: nwin doubles. Returns -3 where a search window leaves the frame, -2 before set_reference. */
+int oracle_periodicity(oracle_cor *c, const uint8_t *frame, int exclude, double *out)
+{
+    if (!c->ready) return -2;
+    const int L = c->L, M = c->M, W = c->W, nt = nthreads_of(c);
+    const int64_t nn = (int64_t)W * W, stride = c->w + 1;
+    for (int k = 0; k < c->nwin; k++) {
+        int sy, sx;
+        if (!search_window(c, k, 0, 0, &sy, &sx)) return -3;
+    }
+    const int lo = py_slice_bound((int64_t)c->R - exclude, M), hi = py_slice_bound((int64_t)c->R + exclude + 1, M);
+    const size_t tsz = (size_t)(c->h + 1) * (c->w + 1);
+    int64_t *sat = (int64_t *)malloc(sizeof(int64_t) * tsz), *sat2 = (int64_t *)malloc(sizeof(int64_t) * tsz);
+    if (!sat || !sat2) {
+        free(sat);
+        free(sat2);
+        return -1;
+    }
+    oracle_summed_area(frame, c->h, c->w, sat, sat2);
+    int failed = 0;
+#pragma omp parallel num_threads(nt)
+    {
+        int64_t *raw = (int64_t *)malloc(sizeof(int64_t) * (size_t)M * M);
+        double *s = (double *)malloc(sizeof(double) * (size_t)M * M);
+        if (!raw || !s) {
+#pragma omp atomic write
+            failed = 1;
+        }
+#pragma omp for schedule(dynamic, 1)
+        for (int k = 0; k < c->nwin; k++) {
+            if (!raw || !s) continue;
+            const uint8_t *t = c->tmpl + (size_t)k * W * W;
+            const double mean = (double)c->tsum[k] / (double)nn;
+            double sq = 0.0;
+            for (int64_t i = 0; i < nn; i++) sq += ((double)t[i] - mean) * ((double)t[i] - mean);
+            if (sqrt(sq) < 1e-6) {
+                out[k] = 1.0;
+                continue;
+            }
+            int sy, sx;
+            search_window(c, k, 0, 0, &sy, &sx);
+            oracle_raw_direct(frame + (size_t)sy * c->w + sx, c->w, L, t, W, W, raw);
+            const int64_t vt = nn * c->tsum2[k] - c->tsum[k] * c->tsum[k];
+            for (int u = 0; u < M; u++)
+                for (int v = 0; v < M; v++)
+                    s[(size_t)u * M + v] = ncc_f64(raw[(size_t)u * M + v], nn, c->tsum[k], vt,
+                                                   box_sum(sat, stride, sy + u, sx + v, W),
+                                                   box_sum(sat2, stride, sy + u, sx + v, W));
+            int any = 0;
+            double best = 0.0;
+            for (int u = 0; u < M; u++)
+                for (int v = 0; v < M; v++) {
+                    if (u >= lo && u < hi && v >= lo && v < hi) continue;
+                    const double x = s[(size_t)u * M + v];
+                    double m = x;
+                    for (int uu = u - 2; uu <= u + 2; uu++)
+                        for (int vv = v - 2; vv <= v + 2; vv++)
+                            if (uu >= 0 && uu < M && vv >= 0 && vv < M && s[(size_t)uu * M + vv] > m)
+                                m = s[(size_t)uu * M + vv];
+                    if (x == m && (!any || x > best)) {
+                        best = x;
+                        any = 1;
+                    }
+                }
+            out[k] = any ? best : 0.0;
+        }
+        free(raw);
+        free(s);
+    }
+    free(sat);
+    free(sat2);
+    return failed ? -1 : 0;
+}
+
 /* the norms of the reference windows (FP32, as the reference) and the integer template moments */
 int oracle_reference_stats(const oracle_cor *c, float *norm, int64_t *tsum, int64_t *tsum2)
 {

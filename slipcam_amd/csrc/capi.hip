@@ -170,6 +170,20 @@ int framecor_summed_area_device(const uint8_t *frames, int n, int height, int wi
     });
 }
 
+int framecor_ncc_surfaces_device(framecor_correlator *c, const uint8_t *frame, int k0, int count, double *surfaces,
+                                 void *stream)
+{
+    if (!c) return fail(FRAMECOR_EINVAL, "framecor: null argument");
+    if (count > 0 && (!frame || !surfaces)) return fail(FRAMECOR_EINVAL, "framecor: null argument");
+    return guarded([&] { c->surfaces(frame, k0, count, surfaces, static_cast<hipStream_t>(stream)); });
+}
+
+int framecor_periodicity(framecor_correlator *c, const uint8_t *frame, int exclude, double *out)
+{
+    if (!c || !frame || !out) return fail(FRAMECOR_EINVAL, "framecor: null argument");
+    return guarded([&] { c->periodicity(frame, exclude, out); });
+}
+
 int framecor_box_sums_device(const uint8_t *frames, int n, int height, int width, int window, int32_t *box,
                              int32_t *box2, void *stream)
 {

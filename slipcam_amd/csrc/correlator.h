@@ -181,6 +181,8 @@ struct framecor_correlator {
     int groupCap = 0;
     // host-call staging
     DeviceBuffer dFrames, dGross, dOffset, dPeak, dSnr, dCov;
+    // NCC surfaces and periodicity: the box maps of one frame, surfaces of a run of windows, results
+    DeviceBuffer dSurfBox, dSurfBox2, dSurf, dPer;
     // per-stage timing: three events per group of the last call
     bool timing = false;
     std::vector<hipEvent_t> events;
@@ -403,6 +405,93 @@ struct framecor_correlator {
         d.shifts = shifts;
         d.ncc = ncc;
         return d;
+    }
+
+    // every window's search window (no gross offset) must lie in the frame
+    void requireSearchInFrame(const char *what) const
+    {
+        for (int y : down)
+            if (y - g.R < 0 || y - g.R + g.L > g.h)
+                invalid(std::string(what) + ": the search window of the windows at row " + std::to_string(y)
+                        + " leaves the frame");
+        for (int x : across)
+            if (x - g.R < 0 || x - g.R + g.L > g.w)
+                invalid(std::string(what) + ": the search window of the windows at column " + std::to_string(x)
+                        + " leaves the frame");
+    }
+
+    // the box maps of one frame on the device, for the surfaces
+    void surfaceBoxes(const uint8_t *frame, hipStream_t s)
+    {
+        dSurfBox.ensure(size_t(g.boxSize) * sizeof(int32_t));
+        dSurfBox2.ensure(size_t(g.boxSize) * sizeof(int32_t));
+        launchBoxSums(frame, 1, g.h, g.w, g.W, dSurfBox.as<int32_t>(), dSurfBox2.as<int32_t>(), nullptr, s);
+    }
+
+    void launchSurfaces(const uint8_t *frame, int k0, int count, double *surf, hipStream_t s)
+    {
+        SurfaceArgs a{};
+        a.g = g;
+        a.wd = windowData();
+        a.frame = frame;
+        a.tmpl = dTmpl.as<uint8_t>();
+        a.box = dSurfBox.as<int32_t>();
+        a.box2 = dSurfBox2.as<int32_t>();
+        a.surf = surf;
+        a.k0 = k0;
+        a.count = count;
+        launchNccSurfaces(a, s);
+    }
+
+    void checkSurfaces(const char *what) const
+    {
+        if (!ready) throw std::runtime_error("framecor: set_reference first");
+        requireSearchInFrame(what);
+        if (surfaceShmem(g) > 160 * 1024)
+            invalid(std::string(what) + ": the search window (window + 2 search = " + std::to_string(g.L)
+                    + ") is too large for the NCC surfaces");
+    }
+
+    // the exact FP64 NCC surfaces (count x M x M) of windows k0 .. k0 + count - 1 against a frame on the
+    // device, each window at its own position (no gross offset)
+    void surfaces(const uint8_t *frame, int k0, int count, double *surf, hipStream_t s)
+    {
+        checkSurfaces("surfaces");
+        if (k0 < 0 || count < 0 || count > g.nwin - k0) invalid("the windows must be windows of the grid");
+        if (count == 0) return;
+        FC_HIP_CHECK(hipSetDevice(device));
+        surfaceBoxes(frame, s);
+        launchSurfaces(frame, k0, count, surf, s);
+    }
+
+    // the second peak of every window's NCC surface against a frame on the host (the periodicity of
+    // python/slipcam/auto_slip.py:105-129 when the frame is the reference): the largest local maximum
+    // outside the square of rows and columns search - exclude .. search + exclude, bounds as a Python
+    // slice of the surface
+    void periodicity(const uint8_t *frame, int exclude, double *out)
+    {
+        checkSurfaces("periodicity");
+        FC_HIP_CHECK(hipSetDevice(device));
+        const int M = g.M;
+        auto bound = [M](int64_t i) {
+            if (i < 0) i = std::max<int64_t>(i + M, 0);
+            return int(std::min<int64_t>(i, M));
+        };
+        const int lo = bound(int64_t(g.R) - exclude), hi = bound(int64_t(g.R) + exclude + 1);
+        const size_t perWindow = size_t(M) * M * sizeof(double);
+        const int chunk = int(std::max<size_t>(1, std::min<size_t>(g.nwin, (size_t(256) << 20) / perWindow)));
+        dFrames.ensure(size_t(g.pixels));
+        dSurf.ensure(size_t(chunk) * perWindow);
+        dPer.ensure(sizeof(double) * g.nwin);
+        FC_HIP_CHECK(hipMemcpyAsync(dFrames.ptr, frame, size_t(g.pixels), hipMemcpyHostToDevice, stream));
+        surfaceBoxes(dFrames.as<uint8_t>(), stream);
+        for (int k0 = 0; k0 < g.nwin; k0 += chunk) {
+            const int cnt = std::min(chunk, g.nwin - k0);
+            launchSurfaces(dFrames.as<uint8_t>(), k0, cnt, dSurf.as<double>(), stream);
+            launchSecondPeak(dSurf.as<double>(), cnt, M, lo, hi, dTvar.as<int64_t>(), k0, dPer.as<double>(), stream);
+        }
+        FC_HIP_CHECK(hipMemcpyAsync(out, dPer.ptr, sizeof(double) * g.nwin, hipMemcpyDeviceToHost, stream));
+        FC_HIP_CHECK(hipStreamSynchronize(stream));
     }
 
     void setReference(const uint8_t *frame)

@@ -810,6 +810,104 @@ const void *kernelNvt(int nvt, int kind, bool queue)
     }
 }
 
+// ---- exact NCC surfaces: the whole M x M FP64 surface of each window against one frame
+//
+// One block of 256 threads per window. The template and the chip are staged in LDS as in the bands;
+// the ceil(M / 16)^2 tiles of 16 x 16 surface points are dealt to the four waves, each tile the K loop
+// of the bands (template row groups i4, column blocks jb) with its B fragments built per tile. Every
+// point is normalized with the exact FP64 NCC of the frame's box maps and written as a double.
+
+constexpr int kSurfThreads = 256;
+
+__global__ void __launch_bounds__(kSurfThreads) nccSurfaces(SurfaceArgs s)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const Geometry &g = s.g;
+    const int k = s.k0 + blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    // the launcher checked that every search window lies in the frame
+    const int sy = s.wd.down[k / g.nx] - g.R, sx = s.wd.across[k % g.nx] - g.R;
+    int8_t *tmpl = reinterpret_cast<int8_t *>(lds);
+    const int tbytes = 4 * s.ni4 * s.tstride;
+    int8_t *chip = tmpl + tbytes;
+    loadTemplate(lds, tbytes + s.crows * s.cstride, tmpl, s.tmpl + size_t(k) * g.W * g.W, g.W, s.tstride, tid,
+                 kSurfThreads);
+    loadChip(chip, s.frame + size_t(sy) * g.w + sx, g.w, g.L, s.cstride, tid, kSurfThreads);
+    __syncthreads();
+
+    Moments mom;
+    mom.n = int64_t(g.W) * g.W;
+    mom.tvar = s.wd.tvar[k];
+    mom.nd = double(mom.n);
+    mom.tau = double(s.wd.tsum[k] - 128 * mom.n);
+    mom.rst = 0.0f;
+    const int M = g.M;
+    double *out = s.surf + size_t(blockIdx.x) * M * M;
+    const int g4 = lane >> 4, m16 = lane & 15;
+    const int bbyte = 16 - m16, bdw = bbyte >> 2, bsh = bbyte & 3;
+    const int nt = s.nt;
+    for (int tile = wave; tile < nt * nt; tile += kSurfThreads / kWave) {
+        const int t = tile / nt, v1 = tile - t * nt;
+        v4i acc = v4i{0, 0, 0, 0};
+        const int8_t *crow = chip + (16 * t + g4 + m16) * s.cstride + 16 * v1;
+        const uint32_t *trow = reinterpret_cast<const uint32_t *>(tmpl + g4 * s.tstride) + bdw;
+        for (int i4 = 0; i4 < s.ni4; i4++) {
+            for (int jb = 0; jb < s.njb; jb++) {
+                const uint32_t *tp = trow + 4 * jb;
+                const uint32_t w0 = tp[0], w1 = tp[1], w2 = tp[2], w3 = tp[3], w4 = tp[4];
+                const v4i b = v4i{int(__builtin_amdgcn_alignbyte(w1, w0, bsh)), int(__builtin_amdgcn_alignbyte(w2, w1, bsh)),
+                                  int(__builtin_amdgcn_alignbyte(w3, w2, bsh)), int(__builtin_amdgcn_alignbyte(w4, w3, bsh))};
+                const v4i av = *reinterpret_cast<const v4i *>(crow + 16 * jb);
+                acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b, acc, 0, 0, 0);
+            }
+            crow += 4 * s.cstride;
+            trow += s.tstride;
+        }
+        const int v = 16 * v1 + m16;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int u = 16 * t + 4 * g4 + r;
+            if (u < M && v < M) {
+                const int64_t b = int64_t(sy + u) * g.bw + sx + v;
+                out[u * M + v] = exactNcc(acc[r], s.box[b], s.box2[b], mom);
+            }
+        }
+    }
+}
+
+// One block per surface: the local maxima outside the excluded square, their largest value
+constexpr int kPeakThreads = 256;
+
+__global__ void __launch_bounds__(kPeakThreads) secondPeak(const double *surf, int M, int lo, int hi,
+                                                           const int64_t *tvar, int k0, double *out)
+{
+    __shared__ double part[kPeakThreads / kWave];
+    const double *S = surf + size_t(blockIdx.x) * M * M;
+    double best = -2.0;   // below every NCC value: no local maximum yet
+    for (int p = threadIdx.x; p < M * M; p += kPeakThreads) {
+        const int u = p / M, v = p - u * M;
+        if (u >= lo && u < hi && v >= lo && v < hi) continue;
+        const double x = S[p];
+        if (!(x > best)) continue;   // cannot raise this thread's best
+        bool peak = true;
+        for (int uu = max(u - 2, 0); uu <= min(u + 2, M - 1) && peak; uu++)
+            for (int vv = max(v - 2, 0); vv <= min(v + 2, M - 1); vv++)
+                if (S[uu * M + vv] > x) {
+                    peak = false;
+                    break;
+                }
+        if (peak) best = x;
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) best = fmax(best, __shfl_xor(best, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kPeakThreads / kWave; w++) best = fmax(best, part[w]);
+        const int k = k0 + blockIdx.x;
+        out[k] = tvar[k] == 0 ? 1.0 : best > -2.0 ? best : 0.0;
+    }
+}
+
 } // namespace
 
 // the instantiated template widths: NJB 16-byte blocks cover templates of W <= 16 NJB - 15
@@ -908,6 +1006,62 @@ void launchInt8Queue(const Int8Kernel &k, const WorkQueue &q, const Int8Args *cl
     const Int8Args *c = clips;
     void *args[] = {&qa, &c};
     FC_HIP_CHECK(hipLaunchKernel(k.func, dim3(blocks), dim3(k.threads), args, k.shmem, stream));
+}
+
+// ---- the surface kernel's layout and launchers
+
+namespace {
+
+struct SurfaceLayout {
+    int njb, ni4, tstride, cstride, crows, nt;
+    size_t shmem;
+};
+
+SurfaceLayout surfaceLayout(const Geometry &g)
+{
+    SurfaceLayout l;
+    l.njb = (g.W + 30) / 16;   // the 16-byte column blocks that meet the template at some shift n < 16
+    l.ni4 = (g.W + 3) / 4;
+    l.nt = (g.M + 15) / 16;
+    l.tstride = 32;
+    while (l.tstride < 16 * (l.njb + 2)) l.tstride += 128;
+    int cs = 16 * (l.nt + l.njb - 1);
+    if (cs < g.L) cs = g.L;
+    cs = (cs + 15) / 16;
+    if (cs % 2 == 0) cs++;
+    l.cstride = 16 * cs;
+    l.crows = 16 * l.nt + 4 * l.ni4;
+    l.shmem = size_t(4 * l.ni4) * l.tstride + size_t(l.crows) * l.cstride;
+    return l;
+}
+
+} // namespace
+
+size_t surfaceShmem(const Geometry &g) { return surfaceLayout(g).shmem; }
+
+void launchNccSurfaces(SurfaceArgs s, hipStream_t stream)
+{
+    if (s.count <= 0) return;
+    const SurfaceLayout l = surfaceLayout(s.g);
+    if (l.shmem > 160 * 1024) throw std::invalid_argument("framecor: the search window is too large for the NCC surfaces");
+    s.njb = l.njb;
+    s.ni4 = l.ni4;
+    s.tstride = l.tstride;
+    s.cstride = l.cstride;
+    s.crows = l.crows;
+    s.nt = l.nt;
+    allowLargeLds(reinterpret_cast<const void *>(nccSurfaces));
+    void *args[] = {&s};
+    FC_HIP_CHECK(hipLaunchKernel(reinterpret_cast<const void *>(nccSurfaces), dim3(s.count), dim3(kSurfThreads), args,
+                                 l.shmem, stream));
+}
+
+void launchSecondPeak(const double *surf, int count, int M, int lo, int hi, const int64_t *tvar, int k0, double *out,
+                      hipStream_t stream)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(secondPeak, dim3(count), dim3(kPeakThreads), 0, stream, surf, M, lo, hi, tvar, k0, out);
+    FC_HIP_CHECK(hipGetLastError());
 }
 
 } // namespace framecor

@@ -82,6 +82,29 @@ void fillInt8Args(Int8Args &a, const Int8Layout &l);
 // the persistent form over a work queue; clips: device array of the clips' arguments (layout filled)
 void launchInt8Queue(const Int8Kernel &k, const WorkQueue &q, const Int8Args *clips, int blocks, hipStream_t stream);
 
+// ---- exact NCC surfaces and the second peak (int8_corr.hip)
+
+struct SurfaceArgs {
+    Geometry g;
+    WindowData wd;
+    const uint8_t *frame;        // h x w: each window is correlated with its search window here (no gross)
+    const uint8_t *tmpl;         // nwin x W x W, the reference windows
+    const int32_t *box, *box2;   // bh x bw box maps of the frame
+    double *surf;                // count x M x M, the exact FP64 NCC surfaces
+    int k0, count;               // windows k0 .. k0 + count - 1
+    // set by the launcher: template column blocks and row groups, LDS strides and rows, tiles per side
+    int njb, ni4, tstride, cstride, crows, nt;
+};
+
+// the LDS of the surface kernel (template rows and the chip); above 160 KB it is not supported
+size_t surfaceShmem(const Geometry &g);
+void launchNccSurfaces(SurfaceArgs s, hipStream_t stream);
+// For i < count: out[k0 + i] = the largest value of surf[i] (M x M) at a local maximum (a point equal to
+// the largest value of its 5 x 5 neighbourhood clipped to the surface) outside the square of rows and
+// columns lo .. hi - 1; 0 where there is none; 1 where window k0 + i's template is flat (tvar 0)
+void launchSecondPeak(const double *surf, int count, int M, int lo, int hi, const int64_t *tvar, int k0, double *out,
+                      hipStream_t stream);
+
 // ---- the FP32 FFT form (fft_corr.hip)
 
 constexpr int kMaxFactors = 12;

@@ -75,6 +75,10 @@ def _load():
     lib.framecor_summed_area_device.argtypes = [P, I, I, I, P, P, P]
     lib.framecor_box_sums_device.restype = I
     lib.framecor_box_sums_device.argtypes = [P, I, I, I, I, P, P, P]
+    lib.framecor_ncc_surfaces_device.restype = I
+    lib.framecor_ncc_surfaces_device.argtypes = [P, P, I, I, P, P]
+    lib.framecor_periodicity.restype = I
+    lib.framecor_periodicity.argtypes = [P, P, I, P]
     lib.framecor_batch_create.restype = I
     lib.framecor_batch_create.argtypes = [P, I, I, I, ctypes.POINTER(P)]
     lib.framecor_batch_destroy.restype = None
@@ -304,6 +308,44 @@ class Correlator:
             _check(_lib.framecor_correlate_device(
                 self._c, frames.data_ptr(), n, None if g is None else g.data_ptr(), out['offsets'].data_ptr(),
                 out['peak'].data_ptr(), out['snr'].data_ptr(), out['cov'].data_ptr(), stream))
+        return out
+
+
+    def surfaces(self, frame):
+        """
+        The exact NCC surfaces of every window against a frame (height x width uint8 torch tensor on
+        the GPU), each window at its own position: a float64 tensor down x across x M x M (M = 2 search
+        + 1), [.., u, v] the NCC at shift (u - search, v - search); exact integer sums normalized in
+        FP64, 0 where the search box or the template is flat. Computed on the current stream.
+        """
+        import torch
+        if not self._ready:
+            raise RuntimeError('framecor: set_reference first')
+        if not (_is_device_tensor(frame) and frame.dtype == torch.uint8 and tuple(frame.shape) == (self._h, self._w)):
+            raise ValueError('framecor: the frame must be a height x width uint8 tensor on the GPU')
+        if frame.device.index != self._device:
+            raise ValueError(f'framecor: the frame is on {frame.device}, the correlator on GPU {self._device}')
+        frame = frame.contiguous()
+        ny, nx = self._shape
+        M = 2 * self._search + 1
+        out = torch.empty((ny, nx, M, M), dtype=torch.float64, device=frame.device)
+        stream = torch.cuda.current_stream(frame.device).cuda_stream
+        _check(_lib.framecor_ncc_surfaces_device(self._c, frame.data_ptr(), 0, ny * nx, out.data_ptr(), stream))
+        return out
+
+    def periodicity(self, frame, exclude=5):
+        """The second peak of each window's NCC with its own surroundings in frame (height x width
+        uint8; the reference frame for slipcam-auto's periodicity): down x across float64, 1 for a flat
+        window, 0 where the surface has no local maximum outside the excluded square"""
+        if not self._ready:
+            raise RuntimeError('framecor: set_reference first')
+        if _is_device_tensor(frame):
+            frame = frame.cpu().numpy()
+        frame = np.ascontiguousarray(frame, np.uint8)
+        if frame.ndim != 2 or frame.shape != (self._h, self._w):
+            raise ValueError('framecor: the frame must be height x width')
+        out = np.empty(self._shape, np.float64)
+        _check(_lib.framecor_periodicity(self._c, _ptr(frame), int(exclude), _ptr(out)))
         return out
 
 

@@ -298,3 +298,81 @@ def test_golden_fixture_reproduced():
     # the known shifts are recovered by the fixture's outputs
     err = g['direct_offsets'] - g['shifts'][:, None, None, :]
     assert np.median(np.abs(err)) < 0.1
+
+
+# ---- the periodicity of slipcam-auto (python/slipcam/auto_slip.py:105-129)
+
+def periodicity_scipy(ref, y0, x0, window, search, exclude=5):
+    """The reference's periodicity in another formulation of the same mathematics: the correlation by
+    scipy.signal.correlate (FFT), the box sums by cumulative sums, the variance clamped at 1e-6 as the
+    reference does, local maxima by maximum_filter in 'nearest' mode (the same as the reference's
+    'reflect' for a 5-point filter: both extend the surface with values inside it), the excluded square
+    as a numpy slice"""
+    ref = ref.astype(np.float64)
+    n = window * window
+    out = np.ones((len(y0), len(x0)))
+
+    def box(c):
+        return c[window:, window:] - c[:-window, window:] - c[window:, :-window] + c[:-window, :-window]
+
+    for i, y in enumerate(y0):
+        for j, x in enumerate(x0):
+            t = ref[y:y + window, x:x + window]
+            t = t - t.mean()
+            tn = np.sqrt(np.sum(t * t))
+            if tn < 1e-6:
+                continue
+            s = ref[y - search:y + window + search, x - search:x + window + search]
+            num = signal.correlate(s, t, mode='valid', method='fft')
+            c1 = np.pad(s.cumsum(0).cumsum(1), ((1, 0), (1, 0)))
+            c2 = np.pad((s * s).cumsum(0).cumsum(1), ((1, 0), (1, 0)))
+            var = np.maximum(box(c2) - box(c1) ** 2 / n, 1e-6)
+            ncc = num / (np.sqrt(var) * tn)
+            peak = ncc == ndimage.maximum_filter(ncc, size=5, mode='nearest')
+            peak[search - exclude:search + exclude + 1, search - exclude:search + exclude + 1] = False
+            out[i, j] = ncc[peak].max() if peak.any() else 0.0
+    return out
+
+
+def mixed_texture(h=240, w=320, seed=7):
+    """Smoothed noise, a 12 x 12 tile repeated over the lower right quarter and a flat block"""
+    img = synthetic.to_uint8(ref_texture(h, w, seed))
+    tile = np.random.default_rng(seed).integers(0, 256, (12, 12), dtype=np.uint8)
+    img[h // 2:, w // 2:] = np.tile(tile, (h // 24 + 1, w // 24 + 1))[:h - h // 2, :w - w // 2]
+    img[:h // 3, :w // 4] = 77
+    return img
+
+
+@pytest.mark.parametrize('exclude', [5, 0, 16, 20])
+def test_periodicity_matches_scipy(exclude):
+    ref = mixed_texture()
+    W, R = 32, 16
+    down, across = window_grid(240, 320, W, R, 16)
+    cor = OracleCorrelator(240, 320, down, across, window=W, search=R)
+    cor.set_reference(ref)
+    got = cor.periodicity(ref, exclude=exclude)
+    want = periodicity_scipy(ref, down, across, W, R, exclude)
+    assert got.shape == want.shape
+    assert np.abs(got - want).max() < 1e-9
+
+
+def test_periodicity_closed_forms():
+    W, R = 32, 16
+    tile = np.random.default_rng(1).integers(0, 256, (12, 12), dtype=np.uint8)
+    periodic = np.tile(tile, (20, 27))[:240, :320]
+    down, across = window_grid(240, 320, W, R, 24)
+    cor = OracleCorrelator(240, 320, down, across, window=W, search=R)
+    cor.set_reference(periodic)
+    # the window repeats at a shift of 12: its second peak is the NCC of a window with itself
+    assert np.abs(cor.periodicity(periodic) - 1.0).max() < 1e-12
+    # an excluded square covering the surface leaves no local maximum
+    assert np.array_equal(cor.periodicity(periodic, exclude=R), np.zeros(cor.shape))
+    # flat windows are periodic
+    flat = np.full((240, 320), 200, np.uint8)
+    cor.set_reference(flat)
+    assert np.array_equal(cor.periodicity(flat), np.ones(cor.shape))
+    # white noise has no second peak near 1
+    noise = np.random.default_rng(2).integers(0, 256, (240, 320), dtype=np.uint8)
+    cor.set_reference(noise)
+    p = cor.periodicity(noise)
+    assert p.max() < 0.3 and p.min() > 0.0
