@@ -164,6 +164,22 @@ def time_periodicity(d, repeats=3):
             'periodic_windows': int((per > 0.8).sum()), 'api': 'Correlator.periodicity (slipcam_amd.auto_slip.periodicity)'}
 
 
+def time_coarse_track(frames=16):
+    """slipcam-auto's coarse tracking (python/slipcam/stabilize.py:55-96) of a 720 x 1280 video whose
+    camera moves by a known similarity transform per frame: window 96, step 96, search 32, per frame one
+    GPU call for every window's NCC peak and the host's robust fit, through the host API"""
+    import numpy as np
+    from slipcam_amd import stabilize, synthetic
+    v = synthetic.similarity_video(n=frames, height=720, width=1280, ref=frames // 2)
+    stabilize.coarse_track(v['frames'][:3], 1, v['text'])
+    t0 = time.perf_counter()
+    params, used, ngrid = stabilize.coarse_track(v['frames'], v['ref'], v['text'])
+    ms = (time.perf_counter() - t0) * 1e3
+    return {'ms_per_frame': round(ms / (frames - 1), 3), 'frames': frames, 'windows': int(ngrid),
+            'max_shift_error_px': round(float(np.abs(params[:, :2] - v['params'][:, :2]).max()), 4),
+            'api': 'slipcam_amd.stabilize.coarse_track'}
+
+
 def cpu_baseline(d, seconds):
     import oracle
     from slipcam_amd import framecor
@@ -294,6 +310,7 @@ def main():
                      'ops_per_launch': ops_launch},
     }
     line['periodicity'] = time_periodicity(d)
+    line['coarse_track'] = time_coarse_track()
     if not args.no_batch:
         line['batch'] = batch_workload(torch, framecor, synthetic, max(2, args.steps // 2), 1)
     if not args.no_cpu_baseline:

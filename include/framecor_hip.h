@@ -214,14 +214,30 @@ int framecor_box_sums_device(const uint8_t *frames, int n, int height, int width
 int framecor_periodicity(framecor_correlator *c, const uint8_t *frame, int exclude, double *out);
 
 /* The exact FP64 NCC surfaces of windows k0 .. k0 + count - 1 against one frame (h x w uint8, device),
- * each window at its own position with no gross offset: surfaces (count x M x M doubles, device)
- * [i][u][v] = the NCC of window k0 + i with the frame at shift (u - search, v - search), exact integer
- * sums normalized in FP64 as (W^2 sum FT - sum T sum F) / sqrt((W^2 sum T^2 - (sum T)^2)
- * (W^2 sum F^2 - (sum F)^2)), 0 where either variance is 0 (cxx/surface.h:30-35). The surface that
- * cxx/cpu.cpp:138-154 evaluates in FP32 per window before its argmax. Enqueued on stream, not
- * synchronized; one call at a time per correlator (it owns the frame's box maps). */
-int framecor_ncc_surfaces_device(framecor_correlator *c, const uint8_t *frame, int k0, int count, double *surfaces,
-                                 void *stream);
+ * each search window at its window's position plus its gross offset (gross: nwin x 2 int32 (down,
+ * across) on the device, indexed by window, or NULL): surfaces (count x M x M doubles, device)
+ * [i][u][v] = the NCC of window k0 + i with the frame at shift (gross + u - search, gross + v - search),
+ * exact integer sums normalized in FP64 as (W^2 sum FT - sum T sum F) / sqrt((W^2 sum T^2 - (sum T)^2)
+ * (W^2 sum F^2 - (sum F)^2)), 0 where either variance is 0 (cxx/surface.h:30-35); NaN throughout where
+ * the search window leaves the frame. The surface that cxx/cpu.cpp:138-154 evaluates in FP32 per window
+ * before its argmax. Enqueued on stream, not synchronized; one call at a time per correlator (it owns
+ * the frame's box maps). */
+int framecor_ncc_surfaces_device(framecor_correlator *c, const uint8_t *frame, const int32_t *gross, int k0, int count,
+                                 double *surfaces, void *stream);
+
+/* The peak of every window's NCC with one frame (h x w uint8) as python/slipcam/stabilize.py:22-45
+ * `ncc_peak`, for every window of the grid at once: on the exact FP64 surface (above; gross: nwin x 2
+ * int32 or NULL), the first maximum (u, v) in row-major order, each coordinate refined by the parabola
+ * through the maximum and its two neighbours (0 at the surface's border or where the parabola's
+ * curvature is below 1e-12): offsets[2k], offsets[2k + 1] = u + du - search, v + dv - search (doubles,
+ * relative to the window's position plus its gross offset); peak[k] = the maximum. NaN, NaN and 0 for a
+ * flat template or a search window leaving the frame. Host arrays, synchronous. */
+int framecor_ncc_peaks(framecor_correlator *c, const uint8_t *frame, const int32_t *gross, double *offsets,
+                       double *peak);
+
+/* The same with device arrays, enqueued on stream, not synchronized. */
+int framecor_ncc_peaks_device(framecor_correlator *c, const uint8_t *frame, const int32_t *gross, double *offsets,
+                              double *peak, void *stream);
 
 #ifdef __cplusplus
 }

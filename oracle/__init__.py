@@ -49,6 +49,8 @@ def lib():
     L.oracle_correlate_fft.argtypes = [P, P, I, P, P, P, P, P, P]
     L.oracle_correlate_direct.restype = I
     L.oracle_correlate_direct.argtypes = [P, P, I, P, P, P, P, P, P, P]
+    L.oracle_ncc_peaks.restype = I
+    L.oracle_ncc_peaks.argtypes = [P, P, P, P, P]
     L.oracle_periodicity.restype = I
     L.oracle_periodicity.argtypes = [P, P, I, P]
     L.oracle_reference_stats.restype = I
@@ -215,6 +217,20 @@ class OracleCorrelator:
             if rc:
                 raise RuntimeError(f'oracle_correlate_fft: {rc}')
         return out
+
+    def ncc_peaks(self, frame, gross=None):
+        """The ncc_peak of slipcam's stabilize.py for every window against frame: (offsets down x across
+        x 2, peak down x across), float64; gross: down x across x 2 integer offsets or None"""
+        frame = np.ascontiguousarray(frame, np.uint8)
+        assert frame.shape == (self.h, self.w)
+        g = None if gross is None else np.ascontiguousarray(gross, np.int32)
+        assert g is None or g.shape == self.shape + (2,)
+        off = np.empty(self.shape + (2,), np.float64)
+        peak = np.empty(self.shape, np.float64)
+        rc = lib().oracle_ncc_peaks(self._c, _ptr(frame), _ptr(g), _ptr(off), _ptr(peak))
+        if rc:
+            raise RuntimeError(f'oracle_ncc_peaks: {rc}')
+        return off, peak
 
     def periodicity(self, frame, exclude=5):
         """The second peak of each window's exact NCC with its own surroundings in frame (down x across

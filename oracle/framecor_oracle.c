@@ -867,6 +867,81 @@ int oracle_periodicity(oracle_cor *c, const uint8_t *frame, int exclude, double 
     return failed ? -1 : 0;
 }
 
+/* the parabolic refinement of python/slipcam/stabilize.py:37-39: the vertex of the parabola through
+ * (-1, m), (0, c), (1, p), 0 where its curvature is below 1e-12 */
+static double parabola_refine(double c, double m, double p)
+{
+    const double den = m - 2 * c + p;
+    return fabs(den) < 1e-12 ? 0.0 : 0.5 * (m - p) / den;
+}
+
+/* The peak of python/slipcam/stabilize.py:22-45 ncc_peak for every window of the grid against one
+ * frame: the NCC of the reference window with its search window (at the window's position plus its
+ * gross offset; the exact NCC of ncc_f64, which the reference evaluates with FP64 FFTs, :29-34), its
+ * first maximum in row-major order (np.argmax, :35-36), each coordinate refined by the parabola
+ * through it and its two neighbours when it is not on the border (:40-41), relative to the centre
+ * (M - 1) / 2 (:42-43). NaN, NaN, 0 for a flat template (tn < 1e-6, :24-28) or a search window leaving
+ * the frame (coarse_track skips those, :85-86). gross: nwin x 2 or NULL; offsets: nwin x 2 (down,
+ * across), peak: nwin. */
+int oracle_ncc_peaks(oracle_cor *c, const uint8_t *frame, const int *gross, double *offsets, double *peak)
+{
+    if (!c->ready) return -2;
+    const int L = c->L, M = c->M, W = c->W, nt = nthreads_of(c);
+    const int64_t nn = (int64_t)W * W, stride = c->w + 1;
+    const size_t tsz = (size_t)(c->h + 1) * (c->w + 1);
+    int64_t *sat = (int64_t *)malloc(sizeof(int64_t) * tsz), *sat2 = (int64_t *)malloc(sizeof(int64_t) * tsz);
+    if (!sat || !sat2) {
+        free(sat);
+        free(sat2);
+        return -1;
+    }
+    oracle_summed_area(frame, c->h, c->w, sat, sat2);
+    int failed = 0;
+#pragma omp parallel num_threads(nt)
+    {
+        int64_t *raw = (int64_t *)malloc(sizeof(int64_t) * (size_t)M * M);
+        double *s = (double *)malloc(sizeof(double) * (size_t)M * M);
+        if (!raw || !s) {
+#pragma omp atomic write
+            failed = 1;
+        }
+#pragma omp for schedule(dynamic, 1)
+        for (int k = 0; k < c->nwin; k++) {
+            if (!raw || !s) continue;
+            offsets[2 * k] = offsets[2 * k + 1] = NAN;
+            peak[k] = 0.0;
+            const uint8_t *t = c->tmpl + (size_t)k * W * W;
+            const double mean = (double)c->tsum[k] / (double)nn;
+            double sq = 0.0;
+            for (int64_t i = 0; i < nn; i++) sq += ((double)t[i] - mean) * ((double)t[i] - mean);
+            int sy, sx;
+            if (sqrt(sq) < 1e-6 || !search_window(c, k, gross ? gross[2 * k] : 0, gross ? gross[2 * k + 1] : 0, &sy, &sx))
+                continue;
+            oracle_raw_direct(frame + (size_t)sy * c->w + sx, c->w, L, t, W, W, raw);
+            const int64_t vt = nn * c->tsum2[k] - c->tsum[k] * c->tsum[k];
+            int bi = 0;
+            for (int i = 0; i < M * M; i++) {
+                const int u = i / M, v = i % M;
+                s[i] = ncc_f64(raw[i], nn, c->tsum[k], vt, box_sum(sat, stride, sy + u, sx + v, W),
+                               box_sum(sat2, stride, sy + u, sx + v, W));
+                if (s[i] > s[bi]) bi = i;
+            }
+            const int u = bi / M, v = bi % M;
+            const double du = (u > 0 && u < M - 1) ? parabola_refine(s[bi], s[bi - M], s[bi + M]) : 0.0;
+            const double dv = (v > 0 && v < M - 1) ? parabola_refine(s[bi], s[bi - 1], s[bi + 1]) : 0.0;
+            const double c0 = (M - 1) / 2.0;
+            offsets[2 * k] = u + du - c0;
+            offsets[2 * k + 1] = v + dv - c0;
+            peak[k] = s[bi];
+        }
+        free(raw);
+        free(s);
+    }
+    free(sat);
+    free(sat2);
+    return failed ? -1 : 0;
+}
+
 /* the norms of the reference windows (FP32, as the reference) and the integer template moments */
 int oracle_reference_stats(const oracle_cor *c, float *norm, int64_t *tsum, int64_t *tsum2)
 {

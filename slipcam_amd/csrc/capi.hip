@@ -170,12 +170,26 @@ int framecor_summed_area_device(const uint8_t *frames, int n, int height, int wi
     });
 }
 
-int framecor_ncc_surfaces_device(framecor_correlator *c, const uint8_t *frame, int k0, int count, double *surfaces,
-                                 void *stream)
+int framecor_ncc_surfaces_device(framecor_correlator *c, const uint8_t *frame, const int32_t *gross, int k0, int count,
+                                 double *surfaces, void *stream)
 {
     if (!c) return fail(FRAMECOR_EINVAL, "framecor: null argument");
     if (count > 0 && (!frame || !surfaces)) return fail(FRAMECOR_EINVAL, "framecor: null argument");
-    return guarded([&] { c->surfaces(frame, k0, count, surfaces, static_cast<hipStream_t>(stream)); });
+    return guarded([&] { c->surfaces(frame, gross, k0, count, surfaces, static_cast<hipStream_t>(stream)); });
+}
+
+int framecor_ncc_peaks(framecor_correlator *c, const uint8_t *frame, const int32_t *gross, double *offsets,
+                       double *peak)
+{
+    if (!c || !frame || !offsets || !peak) return fail(FRAMECOR_EINVAL, "framecor: null argument");
+    return guarded([&] { c->peaksHost(frame, gross, offsets, peak); });
+}
+
+int framecor_ncc_peaks_device(framecor_correlator *c, const uint8_t *frame, const int32_t *gross, double *offsets,
+                              double *peak, void *stream)
+{
+    if (!c || !frame || !offsets || !peak) return fail(FRAMECOR_EINVAL, "framecor: null argument");
+    return guarded([&] { c->peaksDevice(frame, gross, offsets, peak, static_cast<hipStream_t>(stream)); });
 }
 
 int framecor_periodicity(framecor_correlator *c, const uint8_t *frame, int exclude, double *out)

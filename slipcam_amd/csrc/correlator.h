@@ -428,12 +428,13 @@ struct framecor_correlator {
         launchBoxSums(frame, 1, g.h, g.w, g.W, dSurfBox.as<int32_t>(), dSurfBox2.as<int32_t>(), nullptr, s);
     }
 
-    void launchSurfaces(const uint8_t *frame, int k0, int count, double *surf, hipStream_t s)
+    void launchSurfaces(const uint8_t *frame, const int32_t *gross, int k0, int count, double *surf, hipStream_t s)
     {
         SurfaceArgs a{};
         a.g = g;
         a.wd = windowData();
         a.frame = frame;
+        a.gross = gross;
         a.tmpl = dTmpl.as<uint8_t>();
         a.box = dSurfBox.as<int32_t>();
         a.box2 = dSurfBox2.as<int32_t>();
@@ -443,25 +444,70 @@ struct framecor_correlator {
         launchNccSurfaces(a, s);
     }
 
-    void checkSurfaces(const char *what) const
+    // inFrame: every search window must lie in the frame (no gross offsets)
+    void checkSurfaces(const char *what, bool inFrame) const
     {
         if (!ready) throw std::runtime_error("framecor: set_reference first");
-        requireSearchInFrame(what);
+        if (inFrame) requireSearchInFrame(what);
         if (surfaceShmem(g) > 160 * 1024)
             invalid(std::string(what) + ": the search window (window + 2 search = " + std::to_string(g.L)
                     + ") is too large for the NCC surfaces");
     }
 
     // the exact FP64 NCC surfaces (count x M x M) of windows k0 .. k0 + count - 1 against a frame on the
-    // device, each window at its own position (no gross offset)
-    void surfaces(const uint8_t *frame, int k0, int count, double *surf, hipStream_t s)
+    // device, each search window at its window's position plus its gross offset (nwin x 2 on the device,
+    // or null); NaN surfaces where a search window leaves the frame
+    void surfaces(const uint8_t *frame, const int32_t *gross, int k0, int count, double *surf, hipStream_t s)
     {
-        checkSurfaces("surfaces");
+        checkSurfaces("surfaces", false);
         if (k0 < 0 || count < 0 || count > g.nwin - k0) invalid("the windows must be windows of the grid");
         if (count == 0) return;
         FC_HIP_CHECK(hipSetDevice(device));
         surfaceBoxes(frame, s);
-        launchSurfaces(frame, k0, count, surf, s);
+        launchSurfaces(frame, gross, k0, count, surf, s);
+    }
+
+    int surfaceChunk() const
+    {
+        const size_t perWindow = size_t(g.M) * g.M * sizeof(double);
+        return int(std::max<size_t>(1, std::min<size_t>(g.nwin, (size_t(256) << 20) / perWindow)));
+    }
+
+    // the peak of every window's NCC surface against a frame on the device, as
+    // python/slipcam/stabilize.py:22-45 ncc_peak: offsets (nwin x 2 doubles: down, across, relative to
+    // the window's position plus its gross offset) and peak (nwin doubles), on the device
+    void peaksDevice(const uint8_t *frame, const int32_t *gross, double *offsets, double *peak, hipStream_t s)
+    {
+        checkSurfaces("ncc peaks", false);
+        FC_HIP_CHECK(hipSetDevice(device));
+        const int chunk = surfaceChunk();
+        dSurf.ensure(size_t(chunk) * g.M * g.M * sizeof(double));
+        surfaceBoxes(frame, s);
+        for (int k0 = 0; k0 < g.nwin; k0 += chunk) {
+            const int cnt = std::min(chunk, g.nwin - k0);
+            launchSurfaces(frame, gross, k0, cnt, dSurf.as<double>(), s);
+            launchParabolaPeaks(dSurf.as<double>(), cnt, g.M, dTvar.as<int64_t>(), k0, offsets, peak, s);
+        }
+    }
+
+    // the same with host arrays (synchronous)
+    void peaksHost(const uint8_t *frame, const int32_t *gross, double *offsets, double *peak)
+    {
+        checkSurfaces("ncc peaks", false);
+        FC_HIP_CHECK(hipSetDevice(device));
+        dFrames.ensure(size_t(g.pixels));
+        dPer.ensure(sizeof(double) * 3 * size_t(g.nwin));
+        FC_HIP_CHECK(hipMemcpyAsync(dFrames.ptr, frame, size_t(g.pixels), hipMemcpyHostToDevice, stream));
+        if (gross) {
+            dGross.ensure(sizeof(int32_t) * 2 * size_t(g.nwin));
+            FC_HIP_CHECK(hipMemcpyAsync(dGross.ptr, gross, sizeof(int32_t) * 2 * size_t(g.nwin), hipMemcpyHostToDevice,
+                                        stream));
+        }
+        double *dOff = dPer.as<double>(), *dPk = dOff + 2 * size_t(g.nwin);
+        peaksDevice(dFrames.as<uint8_t>(), gross ? dGross.as<int32_t>() : nullptr, dOff, dPk, stream);
+        FC_HIP_CHECK(hipMemcpyAsync(offsets, dOff, sizeof(double) * 2 * size_t(g.nwin), hipMemcpyDeviceToHost, stream));
+        FC_HIP_CHECK(hipMemcpyAsync(peak, dPk, sizeof(double) * size_t(g.nwin), hipMemcpyDeviceToHost, stream));
+        FC_HIP_CHECK(hipStreamSynchronize(stream));
     }
 
     // the second peak of every window's NCC surface against a frame on the host (the periodicity of
@@ -470,7 +516,7 @@ struct framecor_correlator {
     // slice of the surface
     void periodicity(const uint8_t *frame, int exclude, double *out)
     {
-        checkSurfaces("periodicity");
+        checkSurfaces("periodicity", true);
         FC_HIP_CHECK(hipSetDevice(device));
         const int M = g.M;
         auto bound = [M](int64_t i) {
@@ -478,16 +524,15 @@ struct framecor_correlator {
             return int(std::min<int64_t>(i, M));
         };
         const int lo = bound(int64_t(g.R) - exclude), hi = bound(int64_t(g.R) + exclude + 1);
-        const size_t perWindow = size_t(M) * M * sizeof(double);
-        const int chunk = int(std::max<size_t>(1, std::min<size_t>(g.nwin, (size_t(256) << 20) / perWindow)));
+        const int chunk = surfaceChunk();
         dFrames.ensure(size_t(g.pixels));
-        dSurf.ensure(size_t(chunk) * perWindow);
+        dSurf.ensure(size_t(chunk) * M * M * sizeof(double));
         dPer.ensure(sizeof(double) * g.nwin);
         FC_HIP_CHECK(hipMemcpyAsync(dFrames.ptr, frame, size_t(g.pixels), hipMemcpyHostToDevice, stream));
         surfaceBoxes(dFrames.as<uint8_t>(), stream);
         for (int k0 = 0; k0 < g.nwin; k0 += chunk) {
             const int cnt = std::min(chunk, g.nwin - k0);
-            launchSurfaces(dFrames.as<uint8_t>(), k0, cnt, dSurf.as<double>(), stream);
+            launchSurfaces(dFrames.as<uint8_t>(), nullptr, k0, cnt, dSurf.as<double>(), stream);
             launchSecondPeak(dSurf.as<double>(), cnt, M, lo, hi, dTvar.as<int64_t>(), k0, dPer.as<double>(), stream);
         }
         FC_HIP_CHECK(hipMemcpyAsync(out, dPer.ptr, sizeof(double) * g.nwin, hipMemcpyDeviceToHost, stream));

@@ -825,8 +825,18 @@ __global__ void __launch_bounds__(kSurfThreads) nccSurfaces(SurfaceArgs s)
     const Geometry &g = s.g;
     const int k = s.k0 + blockIdx.x;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-    // the launcher checked that every search window lies in the frame
-    const int sy = s.wd.down[k / g.nx] - g.R, sx = s.wd.across[k % g.nx] - g.R;
+    int gy = 0, gx = 0;
+    if (s.gross) {
+        gy = s.gross[2 * k];
+        gx = s.gross[2 * k + 1];
+    }
+    int sy, sx;
+    if (!searchWindow(g, s.wd.down[k / g.nx], s.wd.across[k % g.nx], gy, gx, sy, sx)) {
+        // the search window leaves the frame (uniform over the block): a surface of NaN
+        double *bad = s.surf + size_t(blockIdx.x) * g.M * g.M;
+        for (int p = tid; p < g.M * g.M; p += kSurfThreads) bad[p] = __builtin_nan("");
+        return;
+    }
     int8_t *tmpl = reinterpret_cast<int8_t *>(lds);
     const int tbytes = 4 * s.ni4 * s.tstride;
     int8_t *chip = tmpl + tbytes;
@@ -873,6 +883,64 @@ __global__ void __launch_bounds__(kSurfThreads) nccSurfaces(SurfaceArgs s)
             }
         }
     }
+}
+
+// One block per surface: the first maximum and its parabolic refinement along each axis (the peak of
+// python/slipcam/stabilize.py:22-45 ncc_peak)
+__device__ inline double parabola(double c, double m, double p)
+{
+    const double den = (m - 2.0 * c) + p;
+    return fabs(den) < 1e-12 ? 0.0 : 0.5 * (m - p) / den;
+}
+
+__global__ void __launch_bounds__(256) parabolaPeaks(const double *surf, int M, const int64_t *tvar, int k0,
+                                                     double *offsets, double *peak)
+{
+    __shared__ double pv[256 / kWave];
+    __shared__ int pi[256 / kWave];
+    const double *S = surf + size_t(blockIdx.x) * M * M;
+    const int k = k0 + blockIdx.x;
+    double best = -3.0;
+    int ib = 0x7fffffff;
+    for (int p = threadIdx.x; p < M * M; p += 256) {
+        const double x = S[p];
+        if (x > best) {   // ascending p: the first of equal values stays
+            best = x;
+            ib = p;
+        }
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const double b2 = __shfl_xor(best, o, kWave);
+        const int i2 = __shfl_xor(ib, o, kWave);
+        if (b2 > best || (b2 == best && i2 < ib)) {
+            best = b2;
+            ib = i2;
+        }
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        pv[threadIdx.x / kWave] = best;
+        pi[threadIdx.x / kWave] = ib;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < 256 / kWave; w++)
+        if (pv[w] > best || (pv[w] == best && pi[w] < ib)) {
+            best = pv[w];
+            ib = pi[w];
+        }
+    if (tvar[k] == 0 || isnan(S[0]) || ib >= M * M) {
+        offsets[2 * k] = __builtin_nan("");
+        offsets[2 * k + 1] = __builtin_nan("");
+        peak[k] = 0.0;
+        return;
+    }
+    const int u = ib / M, v = ib - u * M;
+    const double du = u > 0 && u < M - 1 ? parabola(best, S[ib - M], S[ib + M]) : 0.0;
+    const double dv = v > 0 && v < M - 1 ? parabola(best, S[ib - 1], S[ib + 1]) : 0.0;
+    const double c0 = double(M - 1) / 2.0;
+    offsets[2 * k] = (double(u) + du) - c0;
+    offsets[2 * k + 1] = (double(v) + dv) - c0;
+    peak[k] = best;
 }
 
 // One block per surface: the local maxima outside the excluded square, their largest value
@@ -1054,6 +1122,14 @@ void launchNccSurfaces(SurfaceArgs s, hipStream_t stream)
     void *args[] = {&s};
     FC_HIP_CHECK(hipLaunchKernel(reinterpret_cast<const void *>(nccSurfaces), dim3(s.count), dim3(kSurfThreads), args,
                                  l.shmem, stream));
+}
+
+void launchParabolaPeaks(const double *surf, int count, int M, const int64_t *tvar, int k0, double *offsets,
+                         double *peak, hipStream_t stream)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(parabolaPeaks, dim3(count), dim3(256), 0, stream, surf, M, tvar, k0, offsets, peak);
+    FC_HIP_CHECK(hipGetLastError());
 }
 
 void launchSecondPeak(const double *surf, int count, int M, int lo, int hi, const int64_t *tvar, int k0, double *out,

@@ -87,7 +87,8 @@ void launchInt8Queue(const Int8Kernel &k, const WorkQueue &q, const Int8Args *cl
 struct SurfaceArgs {
     Geometry g;
     WindowData wd;
-    const uint8_t *frame;        // h x w: each window is correlated with its search window here (no gross)
+    const uint8_t *frame;        // h x w: each window is correlated with its search window here
+    const int *gross;            // nwin x 2 integer offsets (down, across) of the search windows, or null
     const uint8_t *tmpl;         // nwin x W x W, the reference windows
     const int32_t *box, *box2;   // bh x bw box maps of the frame
     double *surf;                // count x M x M, the exact FP64 NCC surfaces
@@ -104,6 +105,12 @@ void launchNccSurfaces(SurfaceArgs s, hipStream_t stream);
 // columns lo .. hi - 1; 0 where there is none; 1 where window k0 + i's template is flat (tvar 0)
 void launchSecondPeak(const double *surf, int count, int M, int lo, int hi, const int64_t *tvar, int k0, double *out,
                       hipStream_t stream);
+// For i < count, window k = k0 + i: the first maximum (row-major) (u, v) of surf[i], each coordinate
+// refined by the parabola through it and its two neighbours (0 at the border or where the parabola is
+// flat, |den| < 1e-12), relative to the centre: offsets[2k] = u + du - (M - 1) / 2, offsets[2k + 1] =
+// v + dv - (M - 1) / 2, peak[k] = the maximum; NaN, NaN, 0 for a flat template or a surface of NaN
+void launchParabolaPeaks(const double *surf, int count, int M, const int64_t *tvar, int k0, double *offsets,
+                         double *peak, hipStream_t stream);
 
 // ---- the FP32 FFT form (fft_corr.hip)
 

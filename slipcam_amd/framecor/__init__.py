@@ -76,7 +76,11 @@ def _load():
     lib.framecor_box_sums_device.restype = I
     lib.framecor_box_sums_device.argtypes = [P, I, I, I, I, P, P, P]
     lib.framecor_ncc_surfaces_device.restype = I
-    lib.framecor_ncc_surfaces_device.argtypes = [P, P, I, I, P, P]
+    lib.framecor_ncc_surfaces_device.argtypes = [P, P, P, I, I, P, P]
+    lib.framecor_ncc_peaks.restype = I
+    lib.framecor_ncc_peaks.argtypes = [P, P, P, P, P]
+    lib.framecor_ncc_peaks_device.restype = I
+    lib.framecor_ncc_peaks_device.argtypes = [P, P, P, P, P, P]
     lib.framecor_periodicity.restype = I
     lib.framecor_periodicity.argtypes = [P, P, I, P]
     lib.framecor_batch_create.restype = I
@@ -330,8 +334,34 @@ class Correlator:
         M = 2 * self._search + 1
         out = torch.empty((ny, nx, M, M), dtype=torch.float64, device=frame.device)
         stream = torch.cuda.current_stream(frame.device).cuda_stream
-        _check(_lib.framecor_ncc_surfaces_device(self._c, frame.data_ptr(), 0, ny * nx, out.data_ptr(), stream))
+        _check(_lib.framecor_ncc_surfaces_device(self._c, frame.data_ptr(), None, 0, ny * nx, out.data_ptr(), stream))
         return out
+
+    def ncc_peaks(self, frame, gross=None):
+        """
+        The peak of each window's exact NCC with its search window in frame (height x width uint8,
+        numpy), as python/slipcam/stabilize.py:22-45 ncc_peak for every window at once: the first
+        maximum, refined along each axis by the parabola through its neighbours. gross: down x across x 2
+        integer offsets of the search windows (default none). Returns (offsets down x across x 2 float64:
+        down, across relative to the window's position plus its gross offset; peak down x across
+        float64); NaN offsets and peak 0 for a flat window or a search window leaving the frame.
+        """
+        if not self._ready:
+            raise RuntimeError('framecor: set_reference first')
+        frame = np.ascontiguousarray(frame, np.uint8)
+        if frame.ndim != 2 or frame.shape != (self._h, self._w):
+            raise ValueError('framecor: the frame must be height x width')
+        ny, nx = self._shape
+        g = None
+        if gross is not None:
+            g = np.asarray(gross)
+            if g.shape != (ny, nx, 2):
+                raise ValueError('framecor: gross must be down x across x 2')
+            g = np.ascontiguousarray(g, np.int32)
+        off = np.empty((ny, nx, 2), np.float64)
+        peak = np.empty((ny, nx), np.float64)
+        _check(_lib.framecor_ncc_peaks(self._c, _ptr(frame), _ptr(g), _ptr(off), _ptr(peak)))
+        return off, peak
 
     def periodicity(self, frame, exclude=5):
         """The second peak of each window's NCC with its own surroundings in frame (height x width

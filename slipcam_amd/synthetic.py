@@ -185,3 +185,33 @@ def smooth_shifts(n=128, height=1080, width=1920, window=64, search=32, skip=32,
             box = c[window:, window:] - c[:-window, window:] - c[window:, :-window] + c[:-window, :-window]
             truth[f, :, :, a] = gross[f, a] + box[np.ix_(down, across)] / (window * window)
     return dict(ref=ref, frames=frames, gross=gross, truth=truth, window=window, search=search, skip=skip)
+
+
+def similarity_video(n=8, height=480, width=640, ref=3, seed=8):
+    """The coarse-tracking workload: a texture seen by a camera that moves by a similarity transform per
+    frame (shift up to 8 px, rotation and scale up to 2e-3 about the image centre; frame ref is the
+    reference, params 0), with two burned-in text boxes fixed in the frames. frame_k sampled at
+    apply(params_k, x, y) (centred coordinates) shows the reference at (x, y). Returns a dict with
+    frames (n x h x w uint8), params (n x 4: p, q, a, s), text (h x w bool) and ref."""
+    base = texture(height, width, sigma=2.5, seed=seed)
+    rng = np.random.default_rng(seed)
+    params = np.column_stack([rng.uniform(-8, 8, n), rng.uniform(-8, 8, n), rng.uniform(-2e-3, 2e-3, n),
+                              rng.uniform(-2e-3, 2e-3, n)])
+    params[ref] = 0
+    yy, xx = np.mgrid[0:height, 0:width].astype(np.float64)
+    xc, yc = xx - width / 2, yy - height / 2
+    text = np.zeros((height, width), bool)
+    text[20:60, 30:250] = True
+    text[height - 70:height - 30, width - 300:width - 40] = True
+    noise = np.random.default_rng(2)
+    frames = np.empty((n, height, width), np.uint8)
+    for k, (p, q, a, s) in enumerate(params):
+        # the reference point (x, y) appears at x' = (1 + s) x - a y + p, y' = a x + (1 + s) y + q
+        det = (1 + s) ** 2 + a * a
+        x = ((1 + s) * (xc - p) + a * (yc - q)) / det
+        y = (-a * (xc - p) + (1 + s) * (yc - q)) / det
+        img = ndimage.map_coordinates(base, [y + height / 2, x + width / 2], order=3, mode='reflect')
+        img = img + noise.normal(0.0, 1.0, img.shape)
+        img[text] = np.where((xx[text] // 8 + yy[text] // 8) % 2 == 0, 0.0, 255.0)
+        frames[k] = to_uint8(img)
+    return dict(frames=frames, params=params, text=text, ref=ref)

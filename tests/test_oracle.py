@@ -376,3 +376,74 @@ def test_periodicity_closed_forms():
     cor.set_reference(noise)
     p = cor.periodicity(noise)
     assert p.max() < 0.3 and p.min() > 0.0
+
+
+# ---- the peak of slipcam-auto's coarse tracking (python/slipcam/stabilize.py:22-45 ncc_peak)
+
+def ncc_peak_scipy(template, search):
+    """ncc_peak in another formulation of the same mathematics: scipy.signal.correlate (FFT), box sums
+    by cumulative sums, the 1e-6 variance clamp of the reference, np.argmax, the axis parabolas"""
+    t = template.astype(np.float64)
+    t = t - t.mean()
+    tn = np.sqrt(np.sum(t * t))
+    if tn < 1e-6:
+        return np.nan, np.nan, 0.0
+    s = search.astype(np.float64)
+    W = t.shape[0]
+    c1 = np.pad(s.cumsum(0).cumsum(1), ((1, 0), (1, 0)))
+    c2 = np.pad((s * s).cumsum(0).cumsum(1), ((1, 0), (1, 0)))
+    box = lambda c: c[W:, W:] - c[:-W, W:] - c[W:, :-W] + c[:-W, :-W]  # noqa: E731
+    ncc = signal.correlate(s, t, mode='valid', method='fft') / (
+        np.sqrt(np.maximum(box(c2) - box(c1) ** 2 / (W * W), 1e-6)) * tn)
+    i, j = np.unravel_index(np.argmax(ncc), ncc.shape)
+    M = ncc.shape[0]
+
+    def para(c, m, p):
+        return 0.0 if abs(m - 2 * c + p) < 1e-12 else 0.5 * (m - p) / (m - 2 * c + p)
+    di = para(ncc[i, j], ncc[i - 1, j], ncc[i + 1, j]) if 0 < i < M - 1 else 0.0
+    dj = para(ncc[i, j], ncc[i, j - 1], ncc[i, j + 1]) if 0 < j < M - 1 else 0.0
+    return i + di - (M - 1) / 2, j + dj - (M - 1) / 2, ncc[i, j]
+
+
+def test_ncc_peaks_match_scipy():
+    # no periodic part: a repeated texture has exact ties between equal peaks, which the exact oracle
+    # breaks by the first index and FFT rounding breaks either way
+    ref = synthetic.to_uint8(ref_texture(240, 320, seed=9))
+    ref[:80, :80] = 77
+    frame = synthetic.to_uint8(ndimage.shift(ref.astype(np.float64), (2.3, -1.6), order=3, mode='nearest'))
+    W, R = 32, 16
+    down, across = window_grid(240, 320, W, R, 16)
+    cor = OracleCorrelator(240, 320, down, across, window=W, search=R)
+    cor.set_reference(ref)
+    gross = np.random.default_rng(3).integers(-25, 26, (len(down), len(across), 2))
+    off, peak = cor.ncc_peaks(frame, gross)
+    nan = 0
+    for i, y in enumerate(down):
+        for j, x in enumerate(across):
+            sy, sx = y + gross[i, j, 0] - R, x + gross[i, j, 1] - R
+            if sy < 0 or sx < 0 or sy + W + 2 * R > 240 or sx + W + 2 * R > 320:
+                assert np.isnan(off[i, j]).all() and peak[i, j] == 0.0
+                nan += 1
+                continue
+            dy, dx, pk = ncc_peak_scipy(ref[y:y + W, x:x + W], frame[sy:sy + W + 2 * R, sx:sx + W + 2 * R])
+            if np.isnan(dy):
+                assert np.isnan(off[i, j]).all() and peak[i, j] == 0.0
+                continue
+            assert abs(off[i, j, 0] - dy) < 1e-7 and abs(off[i, j, 1] - dx) < 1e-7, (i, j)
+            assert abs(peak[i, j] - pk) < 1e-9
+    assert 0 < nan < off.shape[0] * off.shape[1]
+
+
+def test_ncc_peaks_known_shift():
+    """A frame that is the reference moved by whole pixels: the peak is 1 at the shift, and the
+    parabolas through a symmetric peak add nothing"""
+    ref = synthetic.to_uint8(ref_texture(240, 320, seed=3))
+    frame = np.roll(ref, (5, -7), axis=(0, 1))
+    W, R = 32, 16
+    down, across = window_grid(240, 320, W, R, 40)
+    cor = OracleCorrelator(240, 320, down, across, window=W, search=R)
+    cor.set_reference(ref)
+    off, peak = cor.ncc_peaks(frame)
+    assert np.abs(peak - 1.0).max() < 1e-12
+    assert np.abs(np.rint(off) - np.array([5, -7])).max() == 0
+    assert np.abs(off - np.array([5, -7])).max() < 0.25
