@@ -13,7 +13,9 @@ FP32 FFT algorithm). The line also carries the per-stage GPU times from hipEvent
 calls, the roofline of the int8 correlation kernel, set_reference times, the PCIe-inclusive rate of the
 host API, and a CPU baseline: the oracle's restatement of the reference CPU algorithm (FP32 FFT,
 OpenMP over windows) on a bounded sample of the same frames. The 'batch' object times the batching
-form: 16 uneven clips in one hipGraph launch per step (framecor.Batch, tuned on its first call).
+form: 16 uneven clips in one hipGraph launch per step (framecor.Batch, tuned on its first call). The
+'periodicity', 'coarse_track' and 'diff' objects time slipcam-auto's periodicity of the reference frame,
+its coarse tracking of a 720p video, and the differentiation form on its 1080p geometry.
 
   python bench.py [--steps K] [--warmup W] [--frames 64] [--no-cpu-baseline]
 """
@@ -180,6 +182,45 @@ def time_coarse_track(frames=16):
             'api': 'slipcam_amd.stabilize.coarse_track'}
 
 
+def time_diff(torch, framecor, synthetic, frames=32, steps=3, repeats=3):
+    """The differentiation form on the 1080 x 1920 geometry of its workload (window 64, search 32, skip 32:
+    1624 windows; 32 of its frames, float32 on the GPU): the NCC of every window at continuous shifts,
+    forward and backward (gradients of the summed NCC in the frames and the shifts), and Gauss-Newton
+    refinement of the shifts, timed per frame"""
+    from slipcam_amd.framecor import autograd
+    d = synthetic.smooth_shifts(n=frames)
+    h, w = d['ref'].shape
+    down, across = framecor.window_grid(h, w, d['window'], d['search'], d['skip'])
+    cor = framecor.Correlator(h, w, down, across, window=d['window'], search=d['search'], method='int8')
+    cor.set_reference(d['ref'])
+    dev = torch.tensor(d['frames'], dtype=torch.float32, device='cuda', requires_grad=True)
+    shifts = torch.zeros((frames, len(down), len(across), 2), dtype=torch.float32, device='cuda',
+                         requires_grad=True)
+    gross = d['gross']
+
+    def fwd_bwd():
+        autograd.ncc_shift(cor, dev, shifts, gross=gross).sum().backward()
+
+    fwd_bwd()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        fwd_bwd()
+    torch.cuda.synchronize()
+    fb = (time.perf_counter() - t0) * 1e3 / repeats
+    with torch.no_grad():
+        autograd.refine(cor, dev.detach(), shifts.detach(), gross=gross, steps=steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            autograd.refine(cor, dev.detach(), shifts.detach(), gross=gross, steps=steps)
+        torch.cuda.synchronize()
+    rf = (time.perf_counter() - t0) * 1e3 / repeats
+    return {'frames': frames, 'windows': len(down) * len(across), 'geometry': f'{h}x{w} W{d["window"]} R{d["search"]}',
+            'forward_backward_ms_per_frame': round(fb / frames, 4), 'refine_ms_per_frame': round(rf / frames, 4),
+            'refine_steps': steps, 'dtype': 'f32'}
+
+
 def cpu_baseline(d, seconds):
     import oracle
     from slipcam_amd import framecor
@@ -210,6 +251,8 @@ def main():
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-batch', action='store_true', help='skip the batching workload (16 uneven clips)')
     p.add_argument('--batch-only', action='store_true', help='time only the batching workload')
+    p.add_argument('--no-extras', action='store_true',
+                   help='skip the periodicity, coarse tracking and differentiation timings')
     p.add_argument('--methods', default='int8,fft', help='the forms to time (the first is the headline)')
     p.add_argument('--copy-bytes', type=int, default=0,
                    help='instead: time a plain device copy of this many bytes (calibrates the HBM counters)')
@@ -309,8 +352,11 @@ def main():
                      'kernel': 'int8Correlate', 'kernel_ms_per_launch': round(i8['kernel_ms_per_launch'], 4),
                      'ops_per_launch': ops_launch},
     }
-    line['periodicity'] = time_periodicity(d)
-    line['coarse_track'] = time_coarse_track()
+    if not args.no_extras:
+        log('bench: periodicity, coarse tracking, differentiation')
+        line['periodicity'] = time_periodicity(d)
+        line['coarse_track'] = time_coarse_track()
+        line['diff'] = time_diff(torch, framecor, synthetic)
     if not args.no_batch:
         line['batch'] = batch_workload(torch, framecor, synthetic, max(2, args.steps // 2), 1)
     if not args.no_cpu_baseline:
