@@ -85,16 +85,16 @@ struct Top2 {
 
 __device__ inline Top2 top2Init() { return Top2{-3.0f, -3.0f, 0x7fffffff, 0}; }
 
+// A lane adds its points in ascending index order, so of equal values the first stays the largest and
+// only a strictly larger value replaces it. Branch-free: the second largest is the larger of the old
+// second and the smaller of (x, the old largest).
 __device__ inline void top2Add(Top2 &t, float x, int idx, int raw)
 {
-    if (x > t.f1 || (x == t.f1 && idx < t.i1)) {
-        t.f2 = t.f1;
-        t.f1 = x;
-        t.i1 = idx;
-        t.r1 = raw;
-    }
-    else if (x > t.f2)
-        t.f2 = x;
+    const bool win = x > t.f1;
+    t.f2 = fmaxf(t.f2, fminf(x, t.f1));
+    t.f1 = fmaxf(t.f1, x);
+    t.i1 = win ? idx : t.i1;
+    t.r1 = win ? raw : t.r1;
 }
 
 // the raw int32 surface of one 16-row band (rows 16 band .. + 15, all NVT v tiles) of a staged chip
