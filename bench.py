@@ -26,6 +26,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -170,7 +172,6 @@ def time_coarse_track(frames=16):
     """slipcam-auto's coarse tracking (python/slipcam/stabilize.py:55-96) of a 720 x 1280 video whose
     camera moves by a known similarity transform per frame: window 96, step 96, search 32, per frame one
     GPU call for every window's NCC peak and the host's robust fit, through the host API"""
-    import numpy as np
     from slipcam_amd import stabilize, synthetic
     v = synthetic.similarity_video(n=frames, height=720, width=1280, ref=frames // 2)
     stabilize.coarse_track(v['frames'][:3], 1, v['text'])
@@ -221,7 +222,23 @@ def time_diff(torch, framecor, synthetic, frames=32, steps=3, repeats=3):
             'refine_steps': steps, 'dtype': 'f32'}
 
 
-def cpu_baseline(d, seconds):
+def _agreement(got, want, valid):
+    """offset, peak and SNR differences of two result dicts over the windows valid in both"""
+    both = valid & np.isfinite(got['offsets'][..., 0]) & np.isfinite(want['offsets'][..., 0])
+    off = np.abs(got['offsets'].astype(np.float64) - want['offsets'])[both]
+    snr = np.abs(got['snr'].astype(np.float64) - want['snr'])[both] / np.maximum(np.abs(want['snr'][both]), 1e-30)
+    return {'windows': int(both.sum()), 'offset_median_abs_px': float(np.median(off)) if off.size else None,
+            'offset_max_abs_px': float(off.max()) if off.size else None,
+            'peak_max_abs': float(np.abs(got['peak'].astype(np.float64) - want['peak'])[both].max()) if off.size else None,
+            'snr_max_rel': float(snr.max()) if snr.size else None,
+            'validity_mismatches': int((np.isfinite(got['offsets'][..., 0]) != np.isfinite(want['offsets'][..., 0])).sum())}
+
+
+def cpu_baseline(d, seconds, gpu=None):
+    """The oracle restatement of the reference CPU algorithm timed on a bounded sample of the workload;
+    gpu: the GPU forms' results on the same frames ({'fft': dict, 'int8': dict}), which this leg also
+    checks -- the FFT form against the restatement (the reference's own path) on the timed frames, the
+    int8 form against the FP64 direct-NCC oracle on 48 sampled windows of 16 frames"""
     import oracle
     from slipcam_amd import framecor
     h, w = d['frames'].shape[1:]
@@ -230,15 +247,36 @@ def cpu_baseline(d, seconds):
     o = oracle.OracleCorrelator(h, w, down, across, window=W, search=R)
     o.set_reference(d['ref'])
     done, t0 = 0, time.perf_counter()
+    outs = []
     while done < len(d['frames']) and (done == 0 or time.perf_counter() - t0 < seconds):
-        o.correlate_fft(d['frames'][done:done + 1])
+        outs.append(o.correlate_fft(d['frames'][done:done + 1]))
         done += 1
     elapsed = time.perf_counter() - t0
     nwin = len(down) * len(across)
-    return dict(value=round(nwin * done / elapsed, 1), unit='windows/s', cores=oracle.threads(), kind='port',
+    line = dict(value=round(nwin * done / elapsed, 1), unit='windows/s', cores=oracle.threads(), kind='port',
                 ms_per_frame=round(elapsed / done * 1e3, 2),
                 sample=f'{done} frames x {nwin} windows of the same workload, the oracle restatement of the '
                        f'reference CPU algorithm (own FP32 FFT, no FFTW), OpenMP over windows')
+    if gpu:
+        acc = {}
+        ref_path = {k: np.concatenate([r[k] for r in outs]) for k in outs[0]}
+        if 'fft' in gpu:
+            g = {k: v[:done] for k, v in gpu['fft'].items()}
+            acc['fft_vs_reference_algorithm'] = _agreement(g, ref_path, np.ones(g['peak'].shape, bool))
+            # integer peaks: the rounded offsets agree except where the two FP32 surfaces near-tie
+            both = np.isfinite(g['offsets'][..., 0]) & np.isfinite(ref_path['offsets'][..., 0])
+            acc['fft_vs_reference_algorithm']['integer_peak_mismatches'] = int(
+                (np.rint(g['offsets'][both]) != np.rint(ref_path['offsets'][both])).any(-1).sum())
+        if 'int8' in gpu:
+            n8 = min(16, len(d['frames']))
+            sample = np.random.default_rng(11).choice(nwin, 48, replace=False)
+            want = o.correlate_direct(d['frames'][:n8], windows=sample)
+            g = {k: v[:n8].reshape((n8, nwin) + v.shape[3:])[:, sample] for k, v in gpu['int8'].items()}
+            wnt = {k: v.reshape((n8, nwin) + v.shape[3:])[:, sample] for k, v in want.items()}
+            acc['int8_vs_fp64_oracle'] = _agreement(g, wnt, np.ones(g['peak'].shape, bool))
+            acc['int8_vs_fp64_oracle']['sample'] = f'48 windows x {n8} frames'
+        line['accuracy'] = acc
+    return line
 
 
 def main():
@@ -361,7 +399,8 @@ def main():
         line['batch'] = batch_workload(torch, framecor, synthetic, max(2, args.steps // 2), 1)
     if not args.no_cpu_baseline:
         log('bench: CPU baseline')
-        line['cpu_baseline'] = cpu_baseline(d, args.cpu_seconds)
+        gpu_out = {m: res[m]['cor'].correlate(d['frames']) for m in ('fft', 'int8')}
+        line['cpu_baseline'] = cpu_baseline(d, args.cpu_seconds, gpu_out)
     print(json.dumps(line), flush=True)
 
 

@@ -315,10 +315,12 @@ struct framecor_batch {
             }
             for (Shape &sh : shapes) {
                 if (sh.tuned) continue;
-                // a sample of about 64 items per CU (at most the stage's frames of each clip)
+                // a sample of about 64 items per CU, and at least as many frames as the largest frames per
+                // item (else every candidate of fpb above the sample's frames runs the same work, and
+                // noise picks among them); at most the stage's frames of each clip
                 int64_t perFrame = 0;
                 for (int c : sh.clips) perFrame += clips[c]->g.nwin;
-                const int cap = int(std::max<int64_t>(1, std::min<int64_t>(64, (int64_t(cus) * 64 + perFrame - 1) / perFrame)));
+                const int cap = int(std::max<int64_t>(16, std::min<int64_t>(64, (int64_t(cus) * 64 + perFrame - 1) / perFrame)));
                 std::vector<int> variants;
                 if (isInt8()) {
                     if (sh.layout.edge) variants.push_back(int(Int8Variant::Simple));
