@@ -296,7 +296,7 @@ def main():
                    help='instead: time a plain device copy of this many bytes (calibrates the HBM counters)')
     args = p.parse_args()
     if args.gpus != 1:
-        print('bench.py: this build measures one GPU only (the correlator is single-device; frames or '
+        print('bench.py: the correlator runs on one GPU (it is single-device; frames or '
               'windows would shard across GPUs with no collective on the hot loop). Run with --gpus 1.',
               file=sys.stderr)
         sys.exit(2)
