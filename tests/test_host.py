@@ -143,3 +143,39 @@ def test_product_never_uses_the_oracle():
         if path.endswith(('.py', '.hip', '.h')):
             text = open(path).read()
             assert 'import oracle' not in text and 'from oracle' not in text and 'liboracle' not in text, path
+
+
+C_CALLER = r'''
+#include <stdio.h>
+#include <string.h>
+#include "framecor_hip.h"
+
+int main(void)
+{
+    int down[3] = {16, 48, 80}, across[2] = {16, 48};
+    int ok = framecor_check_params(256, 320, down, 3, across, 2, 32, 16, FRAMECOR_METHOD_INT8, 4096, 21);
+    int bad = framecor_check_params(256, 320, down, 3, across, 2, 2, 16, FRAMECOR_METHOD_FFT, 4096, 21);
+    printf("%s|%d|%d|%s\n", framecor_version(), ok, bad, framecor_last_error());
+    return 0;
+}
+'''
+
+
+def test_header_is_plain_c_and_a_c_program_calls_the_library(tmp_path):
+    """The C ABI from C: the header compiles as strict C99, and a C program linked against the library
+    checks parameters (no GPU needed) with the reference's statuses and messages"""
+    import shutil
+    import subprocess
+    if shutil.which('gcc') is None:
+        pytest.skip('no C compiler')
+    src = tmp_path / 'caller.c'
+    src.write_text(C_CALLER)
+    exe = tmp_path / 'caller'
+    libdir = os.path.dirname(framecor._LIB_PATH)
+    subprocess.run(['gcc', '-std=c99', '-pedantic', '-Wall', '-Werror', '-I', os.path.join(ROOT, 'include'), str(src),
+                    '-o', str(exe), '-L', libdir, '-l:' + os.path.basename(framecor._LIB_PATH), '-Wl,-rpath,' + libdir],
+                   check=True, capture_output=True, text=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.strip()
+    version, ok, bad, msg = out.split('|', 3)
+    assert version == framecor.__version__ and int(ok) == 0 and int(bad) == 1
+    assert msg == 'framecor: the window must be at least 3 pixels'
