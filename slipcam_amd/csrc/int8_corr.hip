@@ -619,8 +619,19 @@ __device__ __attribute__((always_inline)) inline void int8Window(const Int8Args 
 // The register budget of a block of NVT + EW waves: blocks of more than 8 waves run one per CU (no
 // limit below the 512 registers of a lone wave); a template of 112 or more columns (NJB >= 8) leaves
 // LDS for at most three blocks per CU, so its waves take up to 168 registers (3 per SIMD) rather than
-// spill at 128; the rest keep 4 waves per SIMD (two blocks of 8 waves per CU)
-constexpr int int8WavesPerEu(int nvt, int njb, int ew) { return nvt + ew > 8 ? 1 : njb >= 8 ? 3 : 4; }
+// spill at 128. Blocks of up to 6 waves with smaller templates take 6 waves per SIMD (80 registers:
+// four blocks of 6, six of 4 per CU): their per-frame screen and finishing are latency-bound, and the
+// extra resident blocks hide more of it than the spills of the epilogue cost (-12 % at window 64 and
+// search 32, -20 % at 32 / 32 against 4 waves per SIMD; 7 or 8 per SIMD are slower). Blocks of 7 or 8
+// waves keep 4 per SIMD (two blocks per CU, no spills): at 6 per SIMD they gain at most 4 % while their
+// spills move 5x the kernel's HBM bytes through scratch. The persistent work-queue form keeps 4 per
+// SIMD throughout: its loop over items holds more state across the frames, and there the spills cost
+// more than the extra blocks save (a 16-clip batch: 190 ms against 193 ms).
+constexpr int int8WavesPerEu(int nvt, int njb, int ew)
+{
+    return nvt + ew > 8 ? 1 : njb >= 8 ? 3 : nvt + ew <= 6 ? 6 : 4;
+}
+constexpr int int8QueueWavesPerEu(int nvt, int njb, int ew) { return nvt + ew > 8 ? 1 : njb >= 8 ? 3 : 4; }
 
 template <int NVT, int NJB, int EW>
 __global__ void __launch_bounds__(64 * (NVT + EW))
@@ -638,7 +649,7 @@ __attribute__((amdgpu_waves_per_eu(int8WavesPerEu(NVT, NJB, EW)))) int8Correlate
 // included). Each block takes the next item of its XCD's queue, then steals from the other XCDs'.
 template <int NVT, int NJB, int EW>
 __global__ void __launch_bounds__(64 * (NVT + EW))
-__attribute__((amdgpu_waves_per_eu(int8WavesPerEu(NVT, NJB, EW)))) int8Queue(WorkQueue q, const Int8Args *__restrict__ clips)
+__attribute__((amdgpu_waves_per_eu(int8QueueWavesPerEu(NVT, NJB, EW)))) int8Queue(WorkQueue q, const Int8Args *__restrict__ clips)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int item[2];
