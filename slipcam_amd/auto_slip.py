@@ -15,7 +15,10 @@ for a unique one; 0 where no local maximum lies outside the square; 1 for a flat
 The reference evaluates the same NCC with FP64 FFTs (scipy.signal.fftconvolve) and clamps the
 variance of the search box at 1e-6; the exact values differ from it by FFT rounding (about 1e-12),
 and a flat search box gives 0 here where the reference divides its FFT rounding by 1e-3. Every search
-window must lie in the frame (the reference's slices silently shrink otherwise): ValueError.
+window must lie in the frame (the reference's slices silently shrink otherwise): ValueError. The
+correlator holding the templates is the int8 form's up to window 129 and search 71, else the FFT
+form's, whose search window (window + 2 search) must be at most 192 with no prime factor above 31:
+other geometries raise ValueError.
 """
 
 import numpy as np

@@ -109,3 +109,16 @@ def test_periodicity_wrapper_and_errors(gpu):
     fresh = framecor.Correlator(240, 320, down, across, window=W, search=R, method='int8')
     with pytest.raises(RuntimeError, match='set_reference'):
         fresh.periodicity(ref)
+
+
+@pytest.mark.parametrize('W,R', [(144, 20), (131, 8)])
+def test_periodicity_windows_beyond_the_int8_tiles(gpu, W, R):
+    """Windows wider than the int8 form's tiles (129): the correlator is the FFT form's, the surfaces
+    come from the same exact integer kernel, against the oracle"""
+    ref = mixed_texture(256, 320, seed=11)
+    down, across = window_grid(256, 320, W, R, 40)
+    got = auto_slip.periodicity(ref, down, across, W, R)
+    ora = OracleCorrelator(256, 320, down, across, window=W, search=R)
+    ora.set_reference(ref)
+    assert got.shape == (len(down), len(across)) and got.size >= 3
+    assert np.array_equal(got, ora.periodicity(ref))
