@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 # per dispatch as rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE read them (one pass each), and the factors
 # that turn them into bytes: what the same counters read for a plain 256 MiB device copy
 # (profiles/counters_int8Correlate_fftCorrelate_720p_64frames.md holds the passes and the commands)
-TRAFFIC_KIB = {'int8': (2.543e6, 6443.0), 'fft': (1.113e7, 2.083e4)}
+TRAFFIC_KIB = {'int8': (1.706e6, 4828.0), 'fft': (1.113e7, 2.083e4)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 TRAFFIC_FRAMES = 64
 

@@ -638,8 +638,10 @@ __global__ void __launch_bounds__(64 * (NVT + EW))
 __attribute__((amdgpu_waves_per_eu(int8WavesPerEu(NVT, NJB, EW)))) int8Correlate(Int8Args a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
+    // run of frames major: an XCD takes one run of frames over a contiguous range of windows, so that
+    // its L2 holds rows of the frames of that run only (3 % faster at 720p than window major)
     const int logical = xcdRemap(blockIdx.x, gridDim.x);
-    const int k = logical / a.nfb, fblock = logical - k * a.nfb;
+    const int fblock = logical / a.g.nwin, k = logical - fblock * a.g.nwin;
     const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n);
     int8Window<NVT, NJB, EW>(a, k, fBegin, fEnd, lds);
 }
