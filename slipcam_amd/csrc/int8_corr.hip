@@ -661,7 +661,7 @@ __attribute__((amdgpu_waves_per_eu(int8QueueWavesPerEu(NVT, NJB, EW)))) int8Queu
         const int it = __builtin_amdgcn_readfirstlane(item[round & 1]);
         if (it < 0) break;
         int c, k, fBegin, fEnd;
-        q.locate(it, c, k, fBegin, fEnd);
+        q.locateRunMajor(it, c, k, fBegin, fEnd);
         int8Window<NVT, NJB, EW>(clips[c], k, fBegin, fEnd, lds);
     }
 }
