@@ -252,3 +252,22 @@ def test_int8_kernel_variants_agree(gpu, small, fault, workload):
             assert np.array_equal(a[key], b[key], equal_nan=True), (other, key)
         assert np.array_equal(ok, b['snr'] > 0), other
         assert np.abs(a['snr'][ok] / b['snr'][ok] - 1).max() < 1e-12, other
+
+
+@pytest.mark.parametrize('n', [37, 48])
+def test_int8_runs_of_frames_agree(gpu, n):
+    """A call of more than 16 frames is dealt to the blocks in runs of at most 16 frames (37: runs of
+    13, 13 and 11; 48: three of 16), run of frames major: the same bits as frame by frame"""
+    h, w, W, R = 480, 640, 32, 16
+    base = synthetic.texture(h, w, sigma=1.5, seed=n)
+    rng = np.random.default_rng(n)
+    frames = np.stack([synthetic.to_uint8(synthetic.warp(base, *rng.uniform(-3, 3, 2)) + rng.normal(0, 2, base.shape))
+                       for _ in range(n)])
+    down, across = framecor.window_grid(h, w, W, R, 16)
+    cor = framecor.Correlator(h, w, down, across, window=W, search=R, method='int8')
+    cor.set_reference(synthetic.to_uint8(base))
+    whole = cor.correlate(frames)
+    one = [cor.correlate(frames[f:f + 1]) for f in range(n)]
+    assert np.isfinite(whole['offsets'][..., 0]).mean() > 0.9
+    for key in whole:
+        assert np.array_equal(whole[key], np.concatenate([o[key] for o in one]), equal_nan=True), key
