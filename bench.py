@@ -22,6 +22,7 @@ its coarse tracking of a 720p video, and the differentiation form on its 1080p g
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -34,12 +35,13 @@ sys.path.insert(0, ROOT)
 METRIC = 'ms/frame + windows/s @720p 2450 win 64² ±48: int8-MFMA vs FP32-FFT'
 INT8_PEAK_TOPS = 5000.0        # MI355X dense int8 MFMA, TOP/s
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector (packed), TFLOP/s
 
 # HBM traffic per launch of the two correlation kernels on this workload at 64 frames per launch, in KiB
 # per dispatch as rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE read them (one pass each), and the factors
 # that turn them into bytes: what the same counters read for a plain 256 MiB device copy
 # (profiles/counters_int8Correlate_fftCorrelate_720p_64frames.md holds the passes and the commands)
-TRAFFIC_KIB = {'int8': (1.706e6, 4828.0), 'fft': (1.113e7, 2.083e4)}
+TRAFFIC_KIB = {'int8': (1.706e6, 4828.0), 'fft': (2.974e6, 7640.0)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 TRAFFIC_FRAMES = 64
 
@@ -343,12 +345,19 @@ def main():
     achieved = ops_launch / (i8['kernel_ms_per_launch'] * 1e-3) / 1e12
     L = W + 2 * R
     spectra_bytes = nwin * L * (L // 2 + 1) * 8
-    fft_bytes_frame = spectra_bytes + d['frames'][0].nbytes + nwin * 7 * 4
+    # the FFT form: a block takes a run of frames of one window (at most 16, and at least 2048 blocks;
+    # the launch rule of the library), so the spectra pass once per run; its FLOPs per window are the
+    # forward and inverse real FFTs (2 x 2.5 L^2 log2 L^2), the spectrum product and the normalization
+    fft_runs = max(-(-int(ff['frames_per_launch']) // 16), -(-2048 // nwin))
+    fft_runs = min(fft_runs, int(ff['frames_per_launch']))
+    fft_bytes_launch = spectra_bytes * fft_runs + ff['frames_per_launch'] * (d['frames'][0].nbytes + nwin * 7 * 4)
+    fft_flops_frame = nwin * (5.0 * L * L * math.log2(L * L) + 6.0 * L * (L // 2 + 1) + 12.0 * M * M)
+    fft_tflops = fft_flops_frame * ff['frames_per_launch'] / (ff['kernel_ms_per_launch'] * 1e-3) / 1e12
     # the int8 kernel's unique HBM bytes per launch: the reference windows, and per frame the frame,
     # its screen map (8 bytes) and sum-of-squares map (4 bytes) per window position, and the outputs
     bh, bw = d['frames'].shape[1] - W + 1, d['frames'].shape[2] - W + 1
     i8_bytes_launch = nwin * W * W + i8['frames_per_launch'] * (d['frames'][0].nbytes + 12 * bh * bw + nwin * 7 * 4)
-    fft_gbs = fft_bytes_frame * ff['frames_per_launch'] / (ff['kernel_ms_per_launch'] * 1e-3) / 1e9
+    fft_gbs = fft_bytes_launch / (ff['kernel_ms_per_launch'] * 1e-3) / 1e9
 
     line = {
         'metric': METRIC,
@@ -373,10 +382,12 @@ def main():
                 'windows_per_s': round(ff['windows_per_s'], 1), 'dtype': 'f32',
                 'stage_ms_per_frame': {k: round(v, 5) for k, v in ff['stage_ms_per_frame'].items()},
                 'kernel_ms_per_launch': round(ff['kernel_ms_per_launch'], 4),
-                'roofline': {'bound': 'hbm', 'achieved': round(fft_gbs, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                             'frac': round(fft_gbs / HBM_PEAK_GBS, 4),
+                'roofline': {'bound': 'fp32', 'achieved': round(fft_tflops, 2), 'peak': FP32_PEAK_TFLOPS,
+                             'unit': 'TFLOP/s', 'frac': round(fft_tflops / FP32_PEAK_TFLOPS, 4),
+                             'flops_per_launch': fft_flops_frame * ff['frames_per_launch'],
+                             'hbm_gbs': round(fft_gbs, 1), 'hbm_frac': round(fft_gbs / HBM_PEAK_GBS, 4),
                              'traffic': traffic_bytes('fft', ff['frames_per_launch']),
-                             'algorithmic_bytes': fft_bytes_frame * ff['frames_per_launch']},
+                             'algorithmic_bytes': fft_bytes_launch},
                 'set_reference_ms': round(ff['set_reference_ms'], 1),
                 'host_api_ms_per_frame': round(ff['host_api_ms_per_frame'], 4)},
         'int8_over_fft': round(ff['ms_per_frame'] / i8['ms_per_frame'], 3),
