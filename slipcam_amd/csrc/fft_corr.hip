@@ -4,8 +4,8 @@
 // its mean (its exact pixel sum, taken while it is loaded) is loaded into LDS as packed row pairs; forward row passes,
 // the split into half spectra, forward column passes; the product with the conjugate of the cached
 // spectrum of the reference window (read per frame, coalesced: from HBM for the first frame of a
-// run, from L2 for the others); inverse
-// column passes, the combine, inverse row passes over the M surface rows only; then, in LDS, the
+// run, from L2 for the others); inverse column passes, the combine, inverse row passes over the M
+// surface rows only; then, in LDS, the
 // normalization of the M x M surface with the box sums (cxx/surface.h:30-35: c / L^2 over
 // norm * sqrt(v / n)), the first maximum, and the reference's FP32 quadratic fit, FP64 SNR and
 // covariance. HBM traffic per window and frame: the chip, the reference spectrum, the box sums of the
