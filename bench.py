@@ -185,7 +185,7 @@ def time_coarse_track(frames=16):
             'api': 'slipcam_amd.stabilize.coarse_track'}
 
 
-def time_diff(torch, framecor, synthetic, frames=32, steps=3, repeats=3):
+def time_diff(torch, framecor, synthetic, frames=32, steps=3, repeats=5):
     """The differentiation form on the 1080 x 1920 geometry of its workload (window 64, search 32, skip 32:
     1624 windows; 32 of its frames, float32 on the GPU): the NCC of every window at continuous shifts,
     forward and backward (gradients of the summed NCC in the frames and the shifts), and Gauss-Newton
@@ -204,21 +204,22 @@ def time_diff(torch, framecor, synthetic, frames=32, steps=3, repeats=3):
     def fwd_bwd():
         autograd.ncc_shift(cor, dev, shifts, gross=gross).sum().backward()
 
-    fwd_bwd()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(repeats):
-        fwd_bwd()
-    torch.cuda.synchronize()
-    fb = (time.perf_counter() - t0) * 1e3 / repeats
-    with torch.no_grad():
-        autograd.refine(cor, dev.detach(), shifts.detach(), gross=gross, steps=steps)
+    def median_ms(fn):
+        # one untimed call, then the median of the timed ones (a call of 32 frames is a few ms: the
+        # median keeps an occasional host-side stall out of the figure)
+        fn()
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
+        ts = []
         for _ in range(repeats):
-            autograd.refine(cor, dev.detach(), shifts.detach(), gross=gross, steps=steps)
-        torch.cuda.synchronize()
-    rf = (time.perf_counter() - t0) * 1e3 / repeats
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(ts))
+
+    fb = median_ms(fwd_bwd)
+    with torch.no_grad():
+        rf = median_ms(lambda: autograd.refine(cor, dev.detach(), shifts.detach(), gross=gross, steps=steps))
     return {'frames': frames, 'windows': len(down) * len(across), 'geometry': f'{h}x{w} W{d["window"]} R{d["search"]}',
             'forward_backward_ms_per_frame': round(fb / frames, 4), 'refine_ms_per_frame': round(rf / frames, 4),
             'refine_steps': steps, 'dtype': 'f32'}
