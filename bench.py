@@ -206,7 +206,7 @@ def time_diff(torch, framecor, synthetic, frames=32, steps=3, repeats=5):
 
     def median_ms(fn):
         # one untimed call, then the median of the timed ones (a call of 32 frames is a few ms: the
-        # median keeps an occasional host-side stall out of the figure)
+        # median keeps an occasional stall of one call out of the figure)
         fn()
         torch.cuda.synchronize()
         ts = []
