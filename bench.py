@@ -185,28 +185,29 @@ def time_coarse_track(frames=16):
             'api': 'slipcam_amd.stabilize.coarse_track'}
 
 
-def time_diff(torch, framecor, synthetic, frames=32, steps=3, repeats=5):
+def time_diff(torch, framecor, synthetic, frames=32, steps=3, repeats=7):
     """The differentiation form on the 1080 x 1920 geometry of its workload (window 64, search 32, skip 32:
     1624 windows; 32 of its frames, float32 on the GPU): the NCC of every window at continuous shifts,
     forward and backward (gradients of the summed NCC in the frames and the shifts), and Gauss-Newton
     refinement of the shifts, timed per frame"""
     from slipcam_amd.framecor import autograd
-    d = synthetic.smooth_shifts(n=frames)
+    d = synthetic.smooth_shifts_device(n=frames, device='cuda')
     h, w = d['ref'].shape
     down, across = framecor.window_grid(h, w, d['window'], d['search'], d['skip'])
     cor = framecor.Correlator(h, w, down, across, window=d['window'], search=d['search'], method='int8')
     cor.set_reference(d['ref'])
-    dev = torch.tensor(d['frames'], dtype=torch.float32, device='cuda', requires_grad=True)
+    dev = d['frames'].to(torch.float32).requires_grad_(True)
     shifts = torch.zeros((frames, len(down), len(across), 2), dtype=torch.float32, device='cuda',
                          requires_grad=True)
-    gross = d['gross']
+    # the integer offsets on the device once, as a caller that keeps them there
+    gross = torch.tensor(d['gross'], dtype=torch.int32, device='cuda')
 
     def fwd_bwd():
-        autograd.ncc_shift(cor, dev, shifts, gross=gross).sum().backward()
+        # the gradients of the summed NCC in the frames and the shifts (not accumulated into .grad)
+        torch.autograd.grad(autograd.ncc_shift(cor, dev, shifts, gross=gross).sum(), [dev, shifts])
 
-    def median_ms(fn):
-        # one untimed call, then the median of the timed ones (a call of 32 frames is a few ms: the
-        # median keeps an occasional stall of one call out of the figure)
+    def timed(fn):
+        # one untimed call, then each call timed alone: mean and median of the same calls
         fn()
         torch.cuda.synchronize()
         ts = []
@@ -215,13 +216,15 @@ def time_diff(torch, framecor, synthetic, frames=32, steps=3, repeats=5):
             fn()
             torch.cuda.synchronize()
             ts.append((time.perf_counter() - t0) * 1e3)
-        return float(np.median(ts))
+        return float(np.mean(ts)), float(np.median(ts))
 
-    fb = median_ms(fwd_bwd)
+    fb = timed(fwd_bwd)
     with torch.no_grad():
-        rf = median_ms(lambda: autograd.refine(cor, dev.detach(), shifts.detach(), gross=gross, steps=steps))
+        rf = timed(lambda: autograd.refine(cor, dev.detach(), shifts.detach(), gross=gross, steps=steps))
     return {'frames': frames, 'windows': len(down) * len(across), 'geometry': f'{h}x{w} W{d["window"]} R{d["search"]}',
-            'forward_backward_ms_per_frame': round(fb / frames, 4), 'refine_ms_per_frame': round(rf / frames, 4),
+            'forward_backward_ms_per_frame': round(fb[1] / frames, 4), 'refine_ms_per_frame': round(rf[1] / frames, 4),
+            'forward_backward_ms_per_frame_mean': round(fb[0] / frames, 4),
+            'refine_ms_per_frame_mean': round(rf[0] / frames, 4), 'calls': repeats,
             'refine_steps': steps, 'dtype': 'f32'}
 
 

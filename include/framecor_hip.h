@@ -155,7 +155,8 @@ typedef struct framecor_clip {
 
 /* a batch of nclips clips correlated with one method on one GPU; each clip's parameters are checked
  * as framecor_create's (FRAMECOR_EINVAL, the message naming the clip). Clips of the same tile shape
- * (int8: window and search range; fft: window + 2 search) share one persistent work-queue kernel. */
+ * (int8: window, search range and SNR box side min(stat_size, 2 search + 1); fft: window + 2 search)
+ * share one persistent work-queue kernel. */
 int framecor_batch_create(const framecor_clip *clips, int nclips, int method, int device, framecor_batch **out);
 
 /* frees the batch and its device memory (NULL is ignored) */

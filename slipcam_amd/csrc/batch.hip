@@ -2,11 +2,11 @@
 // hipGraph launch
 //
 // A batch holds one correlator per clip (correlator.h: its grid, reference windows and box-sum
-// buffers). Clips with the same tile shape -- the int8 form: the same window and search range (one
-// kernel instantiation and LDS layout); the FFT form: the same search-window size L -- share a
-// persistent kernel that drains a work queue of (clip, window, run of frames) items across all of them
-// (WorkQueue in common.h: one range per XCD, then stealing), so small clips fill the GPU together and
-// large ones spread over every CU. A call is: zero the queue counters (a kernel, not a memset node),
+// buffers). Clips with the same tile shape -- the int8 form: the same window, search range and SNR
+// box side min(stat_size, M) (one kernel instantiation and LDS layout); the FFT form: the same
+// search-window size L -- share a persistent kernel that drains a work queue of (clip, window, run of
+// frames) items across all of them (WorkQueue in common.h: one range per XCD, then stealing), so small
+// clips fill the GPU together and large ones spread over every CU. A call is: zero the queue counters (a kernel, not a memset node),
 // then per stage of frames (each clip's frames in groups its box-sum buffers hold) the box-sum maps of
 // every clip and one queue kernel per tile shape. That sequence is captured once into a hipGraph and
 // replayed while the buffers stay the same; the clips' arguments sit in a device table written before
@@ -34,6 +34,7 @@ __global__ void zeroInts(int *p, int n)
 
 struct Shape {
     int W, R, L;
+    int snr;            // the int8 form: the SNR box side min(stat, M), which sizes the kernel's LDS
     std::vector<int> clips;
     Int8Layout layout{};
     // the plan
@@ -475,13 +476,16 @@ int framecor_batch_create(const framecor_clip *clips, int nclips, int method, in
             int s = 0;
             for (; s < int(b->shapes.size()); s++) {
                 const Shape &sh = b->shapes[s];
-                if (b->isInt8() ? (sh.W == g.W && sh.R == g.R) : sh.L == g.L) break;
+                // int8: one kernel and one LDS layout per (window, search range, SNR box side)
+                const int snr = std::min(g.stat, g.M);
+                if (b->isInt8() ? (sh.W == g.W && sh.R == g.R && sh.snr == snr) : sh.L == g.L) break;
             }
             if (s == int(b->shapes.size())) {
                 Shape sh;
                 sh.W = g.W;
                 sh.R = g.R;
                 sh.L = g.L;
+                sh.snr = std::min(g.stat, g.M);
                 if (b->isInt8()) sh.layout = b->clips[c]->i8;
                 b->shapes.push_back(sh);
             }

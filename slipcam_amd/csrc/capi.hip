@@ -89,7 +89,10 @@ int framecor_ncc_forward(framecor_correlator *c, const float *frames, int n, con
     return guarded([&] {
         DiffArgs d = c->diffArgs(frames, n, gross, shifts, ncc);
         FC_HIP_CHECK(hipSetDevice(c->device));
-        launchDiff(d, DiffOp::Forward, static_cast<hipStream_t>(stream));
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        c->after(s);
+        launchDiff(d, DiffOp::Forward, s);
+        c->done(s);
     });
 }
 
@@ -106,7 +109,10 @@ int framecor_ncc_backward(framecor_correlator *c, const float *frames, int n, co
         d.gradFrames = grad_frames;
         d.gradShifts = grad_shifts;
         FC_HIP_CHECK(hipSetDevice(c->device));
-        launchDiff(d, DiffOp::Backward, static_cast<hipStream_t>(stream));
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        c->after(s);
+        launchDiff(d, DiffOp::Backward, s);
+        c->done(s);
     });
 }
 
@@ -121,7 +127,10 @@ int framecor_ncc_refine(framecor_correlator *c, const float *frames, int n, cons
         d.shiftsOut = shifts_out;
         d.steps = steps;
         FC_HIP_CHECK(hipSetDevice(c->device));
-        launchDiff(d, DiffOp::Refine, static_cast<hipStream_t>(stream));
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        c->after(s);
+        launchDiff(d, DiffOp::Refine, s);
+        c->done(s);
     });
 }
 

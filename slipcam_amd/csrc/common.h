@@ -125,9 +125,12 @@ __device__ inline void invalidateOut(int64_t q, float *offset, float *peak, floa
 // (cxx/surface.h:39-45)
 __device__ inline bool searchWindow(const Geometry &g, int y0, int x0, int gy, int gx, int &sy, int &sx)
 {
-    sy = y0 + gy - g.R;
-    sx = x0 + gx - g.R;
-    return sy >= 0 && sx >= 0 && sy + g.L <= g.h && sx + g.L <= g.w;
+    // in int64: gross offsets are any int32
+    const int64_t ly = int64_t(y0) + gy - g.R, lx = int64_t(x0) + gx - g.R;
+    const bool in = ly >= 0 && lx >= 0 && ly + g.L <= g.h && lx + g.L <= g.w;
+    sy = in ? int(ly) : 0;
+    sx = in ? int(lx) : 0;
+    return in;
 }
 
 // wave-wide reductions (64 lanes)

@@ -187,14 +187,44 @@ struct framecor_correlator {
     bool timing = false;
     std::vector<hipEvent_t> events;
     int eventsUsed = 0;
+    // The order of the correlator's work across streams. Every call that enqueues work on the
+    // correlator's buffers (templates, norms, spectra, box maps, staging) first makes its stream wait
+    // for the last such work of any earlier call (after(s): the event lastUse), and records lastUse
+    // behind its own work (done(s)). So set_reference on the correlator's own stream waits for a device
+    // correlate still running on the caller's stream, and device calls on two streams run in call order.
+    // Not under stream capture (a captured call is ordered by its graph).
+    hipEvent_t lastUse = nullptr;
+    bool used = false;
 
     ~framecor_correlator()
     {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
-        if (stream) {
-            (void)hipSetDevice(device);
-            (void)hipStreamDestroy(stream);
+        if (stream || lastUse) (void)hipSetDevice(device);
+        if (used) (void)hipEventSynchronize(lastUse);
+        if (lastUse) (void)hipEventDestroy(lastUse);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    static bool capturing(hipStream_t s)
+    {
+        hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
         }
+        return st != hipStreamCaptureStatusNone;
+    }
+
+    void after(hipStream_t s)
+    {
+        if (used && !capturing(s)) FC_HIP_CHECK(hipStreamWaitEvent(s, lastUse, 0));
+    }
+
+    void done(hipStream_t s)
+    {
+        if (capturing(s)) return;
+        FC_HIP_CHECK(hipEventRecord(lastUse, s));
+        used = true;
     }
 
     // checks the parameters and sets up the correlator on its device (before set_reference)
@@ -239,6 +269,7 @@ struct framecor_correlator {
         if (device < 0 || device >= count) invalid("no GPU " + std::to_string(device));
         FC_HIP_CHECK(hipSetDevice(device));
         FC_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        FC_HIP_CHECK(hipEventCreateWithFlags(&lastUse, hipEventDisableTiming));
         dDown.ensure(sizeof(int) * ndown);
         dAcross.ensure(sizeof(int) * nacross);
         FC_HIP_CHECK(hipMemcpy(dDown.ptr, down_, sizeof(int) * ndown, hipMemcpyHostToDevice));
@@ -349,12 +380,14 @@ struct framecor_correlator {
         if (n == 0) return;
         FC_HIP_CHECK(hipSetDevice(device));
         ensureGroup(n);
+        after(s);
         for (int f0 = 0; f0 < n; f0 += groupCap) {
             const int nf = std::min(groupCap, n - f0);
             const size_t q0 = size_t(f0) * g.nwin;
             runGroup(frames + size_t(f0) * g.pixels, nf, gross ? gross + 2 * q0 : nullptr, offset + 2 * q0,
                      peak + q0, snr + q0, cov + 3 * q0, s);
         }
+        done(s);
     }
 
     void correlateHost(const uint8_t *frames, int n, const int32_t *gross, float *offset, float *peak, float *snr,
@@ -373,6 +406,7 @@ struct framecor_correlator {
         dSnr.ensure(size_t(G) * g.nwin * sizeof(float));
         dCov.ensure(size_t(G) * g.nwin * 3 * sizeof(float));
         if (gross) dGross.ensure(size_t(G) * g.nwin * 2 * sizeof(int32_t));
+        after(stream);
         for (int f0 = 0; f0 < n; f0 += G) {
             const int nf = std::min(G, n - f0);
             const size_t q0 = size_t(f0) * g.nwin, pairs = size_t(nf) * g.nwin;
@@ -389,6 +423,7 @@ struct framecor_correlator {
             FC_HIP_CHECK(hipMemcpyAsync(cov + 3 * q0, dCov.ptr, pairs * 3 * sizeof(float), hipMemcpyDeviceToHost, stream));
             FC_HIP_CHECK(hipStreamSynchronize(stream));
         }
+        done(stream);
     }
 
     DiffArgs diffArgs(const float *frames, int n, const int32_t *gross, const float *shifts, float *ncc) const
@@ -421,10 +456,9 @@ struct framecor_correlator {
     }
 
     // the box maps of one frame on the device, for the surfaces
+    // (dSurfBox, dSurfBox2: ensured by the caller, before it orders its stream)
     void surfaceBoxes(const uint8_t *frame, hipStream_t s)
     {
-        dSurfBox.ensure(size_t(g.boxSize) * sizeof(int32_t));
-        dSurfBox2.ensure(size_t(g.boxSize) * sizeof(int32_t));
         launchBoxSums(frame, 1, g.h, g.w, g.W, dSurfBox.as<int32_t>(), dSurfBox2.as<int32_t>(), nullptr, s);
     }
 
@@ -463,8 +497,12 @@ struct framecor_correlator {
         if (k0 < 0 || count < 0 || count > g.nwin - k0) invalid("the windows must be windows of the grid");
         if (count == 0) return;
         FC_HIP_CHECK(hipSetDevice(device));
+        dSurfBox.ensure(size_t(g.boxSize) * sizeof(int32_t));
+        dSurfBox2.ensure(size_t(g.boxSize) * sizeof(int32_t));
+        after(s);
         surfaceBoxes(frame, s);
         launchSurfaces(frame, gross, k0, count, surf, s);
+        done(s);
     }
 
     int surfaceChunk() const
@@ -482,12 +520,16 @@ struct framecor_correlator {
         FC_HIP_CHECK(hipSetDevice(device));
         const int chunk = surfaceChunk();
         dSurf.ensure(size_t(chunk) * g.M * g.M * sizeof(double));
+        dSurfBox.ensure(size_t(g.boxSize) * sizeof(int32_t));
+        dSurfBox2.ensure(size_t(g.boxSize) * sizeof(int32_t));
+        after(s);
         surfaceBoxes(frame, s);
         for (int k0 = 0; k0 < g.nwin; k0 += chunk) {
             const int cnt = std::min(chunk, g.nwin - k0);
             launchSurfaces(frame, gross, k0, cnt, dSurf.as<double>(), s);
             launchParabolaPeaks(dSurf.as<double>(), cnt, g.M, dTvar.as<int64_t>(), k0, offsets, peak, s);
         }
+        done(s);
     }
 
     // the same with host arrays (synchronous)
@@ -497,9 +539,10 @@ struct framecor_correlator {
         FC_HIP_CHECK(hipSetDevice(device));
         dFrames.ensure(size_t(g.pixels));
         dPer.ensure(sizeof(double) * 3 * size_t(g.nwin));
+        if (gross) dGross.ensure(sizeof(int32_t) * 2 * size_t(g.nwin));
+        after(stream);
         FC_HIP_CHECK(hipMemcpyAsync(dFrames.ptr, frame, size_t(g.pixels), hipMemcpyHostToDevice, stream));
         if (gross) {
-            dGross.ensure(sizeof(int32_t) * 2 * size_t(g.nwin));
             FC_HIP_CHECK(hipMemcpyAsync(dGross.ptr, gross, sizeof(int32_t) * 2 * size_t(g.nwin), hipMemcpyHostToDevice,
                                         stream));
         }
@@ -508,6 +551,7 @@ struct framecor_correlator {
         FC_HIP_CHECK(hipMemcpyAsync(offsets, dOff, sizeof(double) * 2 * size_t(g.nwin), hipMemcpyDeviceToHost, stream));
         FC_HIP_CHECK(hipMemcpyAsync(peak, dPk, sizeof(double) * size_t(g.nwin), hipMemcpyDeviceToHost, stream));
         FC_HIP_CHECK(hipStreamSynchronize(stream));
+        done(stream);
     }
 
     // the second peak of every window's NCC surface against a frame on the host (the periodicity of
@@ -528,6 +572,9 @@ struct framecor_correlator {
         dFrames.ensure(size_t(g.pixels));
         dSurf.ensure(size_t(chunk) * M * M * sizeof(double));
         dPer.ensure(sizeof(double) * g.nwin);
+        dSurfBox.ensure(size_t(g.boxSize) * sizeof(int32_t));
+        dSurfBox2.ensure(size_t(g.boxSize) * sizeof(int32_t));
+        after(stream);
         FC_HIP_CHECK(hipMemcpyAsync(dFrames.ptr, frame, size_t(g.pixels), hipMemcpyHostToDevice, stream));
         surfaceBoxes(dFrames.as<uint8_t>(), stream);
         for (int k0 = 0; k0 < g.nwin; k0 += chunk) {
@@ -537,6 +584,7 @@ struct framecor_correlator {
         }
         FC_HIP_CHECK(hipMemcpyAsync(out, dPer.ptr, sizeof(double) * g.nwin, hipMemcpyDeviceToHost, stream));
         FC_HIP_CHECK(hipStreamSynchronize(stream));
+        done(stream);
     }
 
     void setReference(const uint8_t *frame)
@@ -575,6 +623,8 @@ struct framecor_correlator {
         dNorm.ensure(sizeof(float) * nwin);
         dTsum.ensure(sizeof(int64_t) * nwin);
         dTvar.ensure(sizeof(int64_t) * nwin);
+        // the templates may still be read by an earlier call's kernels on another stream
+        after(stream);
         FC_HIP_CHECK(hipMemcpyAsync(dTmpl.ptr, tmpl.data(), tmpl.size(), hipMemcpyHostToDevice, stream));
         FC_HIP_CHECK(hipMemcpyAsync(dNorm.ptr, norm.data(), sizeof(float) * nwin, hipMemcpyHostToDevice, stream));
         FC_HIP_CHECK(hipMemcpyAsync(dTsum.ptr, tsum.data(), sizeof(int64_t) * nwin, hipMemcpyHostToDevice, stream));
@@ -599,6 +649,7 @@ struct framecor_correlator {
         else {
             FC_HIP_CHECK(hipStreamSynchronize(stream));
         }
+        done(stream);
         ready = true;
     }
 };

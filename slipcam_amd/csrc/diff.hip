@@ -48,9 +48,18 @@ __device__ inline Tap tapOf(const DiffArgs &d, int k, int64_t q, float sy, float
     Tap t;
     t.a = sy - fy;
     t.b = sx - fx;
-    t.y = d.wd.down[k / g.nx] + gy + int(fy);
-    t.x = d.wd.across[k % g.nx] + gx + int(fx);
-    t.valid = isfinite(sy) && isfinite(sx) && t.y >= 0 && t.x >= 0 && t.y + g.W + 1 <= g.h && t.x + g.W + 1 <= g.w;
+    t.y = t.x = 0;
+    // the shift's magnitude is checked as a float before any conversion (a shift beyond the frame,
+    // however large, is not valid), and the tap's bounds in int64 (gross offsets are any int32)
+    t.valid = isfinite(sy) && isfinite(sx) && fabsf(fy) <= float(g.h) && fabsf(fx) <= float(g.w);
+    if (!t.valid) return t;
+    const int64_t ty = int64_t(d.wd.down[k / g.nx]) + gy + int64_t(fy);
+    const int64_t tx = int64_t(d.wd.across[k % g.nx]) + gx + int64_t(fx);
+    t.valid = ty >= 0 && tx >= 0 && ty + g.W + 1 <= g.h && tx + g.W + 1 <= g.w;
+    if (t.valid) {
+        t.y = int(ty);
+        t.x = int(tx);
+    }
     return t;
 }
 

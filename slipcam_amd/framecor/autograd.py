@@ -33,8 +33,13 @@ def _prepare(cor, frames, shifts, gross):
         raise ValueError('framecor: shifts must be a float32 tensor frames x down x across x 2 on the frames\' GPU')
     g = None
     if gross is not None:
-        g = torch.as_tensor(np.asarray(gross) if not isinstance(gross, torch.Tensor) else gross)
-        g = g.to(device=frames.device, dtype=torch.int32)
+        if isinstance(gross, torch.Tensor):
+            g = gross.to(device=frames.device, dtype=torch.int32)
+        else:
+            # host offsets: staged in pinned memory and copied without blocking (a pageable copy would
+            # wait for the work already queued on the stream)
+            g = torch.from_numpy(np.ascontiguousarray(gross, np.int32)).pin_memory()
+            g = g.to(device=frames.device, non_blocking=True)
         if tuple(g.shape) == (n, 2):
             g = g[:, None, None, :].expand(n, ny, nx, 2)
         elif tuple(g.shape) != (n, ny, nx, 2):

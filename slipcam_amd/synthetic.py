@@ -187,6 +187,55 @@ def smooth_shifts(n=128, height=1080, width=1920, window=64, search=32, skip=32,
     return dict(ref=ref, frames=frames, gross=gross, truth=truth, window=window, search=search, skip=skip)
 
 
+def smooth_shifts_device(n=128, height=1080, width=1920, window=64, search=32, skip=32, seed=6, device='cuda'):
+    """smooth_shifts made on a torch device (the full 128-frame 1080p workload in seconds): the same
+    texture and reference, integer offsets and sine-product fields drawn the same way, the frames warped
+    bilinearly with reflected borders and noise sigma 2 DN in float64 on the device (so not the same
+    noise bits as smooth_shifts); frames is a torch uint8 tensor on the device, the rest numpy"""
+    import torch
+    base = texture(height, width)
+    ref = to_uint8(base)
+    rng = np.random.default_rng(seed)
+    gross = rng.integers(-20, 21, size=(n, 2)).astype(np.int32)
+    dev = torch.device(device)
+    img = torch.tensor(base, dtype=torch.float64, device=dev)
+    yy = torch.arange(height, dtype=torch.float64, device=dev)[:, None].expand(height, width)
+    xx = torch.arange(width, dtype=torch.float64, device=dev)[None, :].expand(height, width)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+
+    def reflect(i, size):
+        # scipy's 'reflect' (half-sample symmetric) boundary for integer taps
+        i = torch.remainder(i, 2 * size)
+        return torch.where(i >= size, 2 * size - 1 - i, i)
+
+    first = search
+    down = first + skip * np.arange((height - 2 * search - window) // skip)
+    across = first + skip * np.arange((width - 2 * search - window) // skip)
+    frames = torch.empty((n, height, width), dtype=torch.uint8, device=dev)
+    truth = np.empty((n, len(down), len(across), 2))
+    for f in range(n):
+        field = []
+        for axis in range(2):
+            ly, lx = rng.uniform(500, 900, 2)
+            py, px = rng.uniform(0, 2 * np.pi, 2)
+            field.append(0.5 * torch.sin(2 * np.pi * yy / ly + py) * torch.sin(2 * np.pi * xx / lx + px))
+        cy, cx = yy - (gross[f, 0] + field[0]), xx - (gross[f, 1] + field[1])
+        y0, x0 = torch.floor(cy), torch.floor(cx)
+        ay, ax = cy - y0, cx - x0
+        r0, r1 = reflect(y0.long(), height), reflect(y0.long() + 1, height)
+        c0, c1 = reflect(x0.long(), width), reflect(x0.long() + 1, width)
+        moved = ((1 - ay) * ((1 - ax) * img[r0, c0] + ax * img[r0, c1])
+                 + ay * ((1 - ax) * img[r1, c0] + ax * img[r1, c1]))
+        noise = torch.randn((height, width), generator=gen, dtype=torch.float64, device=dev) * 2.0
+        frames[f] = torch.clamp(torch.round(moved + noise), 0, 255).to(torch.uint8)
+        for a in range(2):
+            c = torch.nn.functional.pad(field[a], (1, 0, 1, 0)).cumsum(0).cumsum(1)
+            box = c[window:, window:] - c[:-window, window:] - c[window:, :-window] + c[:-window, :-window]
+            truth[f, :, :, a] = gross[f, a] + box.cpu().numpy()[np.ix_(down, across)] / (window * window)
+    return dict(ref=ref, frames=frames, gross=gross, truth=truth, window=window, search=search, skip=skip)
+
+
 def similarity_video(n=8, height=480, width=640, ref=3, seed=8):
     """The coarse-tracking workload: a texture seen by a camera that moves by a similarity transform per
     frame (shift up to 8 px, rotation and scale up to 2e-3 about the image centre; frame ref is the

@@ -10,6 +10,8 @@ tuning, and on replays of the captured graph.
 import numpy as np
 import pytest
 
+from oracle import OracleCorrelator
+from parity import assert_fft_parity, assert_int8_parity, flat_select
 from slipcam_amd import framecor, synthetic
 
 pytestmark = pytest.mark.gpu
@@ -42,7 +44,7 @@ def videos(specs):
 
 def solo(spec, ref, frames, method, variant='auto', gross=None):
     cor = framecor.Correlator(spec['height'], spec['width'], spec['down'], spec['across'], window=spec['window'],
-                              search=spec['search'], method=method)
+                              search=spec['search'], method=method, stat_size=spec.get('stat_size', 21))
     if method == 'int8':
         cor.set_variant(variant)
     cor.set_reference(ref)
@@ -147,3 +149,54 @@ def test_host_arrays(gpu):
     host = batch.correlate([fr.cpu().numpy() for _, fr in vids])
     for c in range(len(specs)):
         same(host[c], dev[c], f'clip {c}')
+
+
+def test_stat_size_per_clip(gpu):
+    """Clips of one window and search range with different SNR windows (21, an even 8, 3, and 40, larger
+    than the 33 x 33 surface): each kernel launch sized for its clips' SNR box; the same bits as each clip
+    alone"""
+    base = synthetic.clip_specs(n=1, seed=91, frames=(3, 3), resolutions=((480, 640),), windows=(32,),
+                                searches=(16,))[0]
+    specs = [dict(base, stat_size=st, seed=base['seed'] + i) for i, st in enumerate((21, 8, 3, 40))]
+    vids = videos(specs)
+    batch, out = run_batch(specs, vids, 'int8')
+    assert batch.info()['shapes'] == 4
+    for c, (s, (ref, fr)) in enumerate(zip(specs, vids)):
+        same(out[c], solo(s, ref, fr, 'int8', batch.plan(c)['variant']), f'clip {c} stat {s["stat_size"]}')
+
+
+@pytest.mark.parametrize('method', ['int8', 'fft'])
+def test_batching_workload_16_clips(gpu, method):
+    """The batching workload itself: 16 seeded clips (480p-1080p, windows 32-128, search 16-64, 8-256 frames)
+    in one graph launch (tuned: int8), each clip's outputs the bits of that clip alone through the
+    Correlator, and per clip 6 windows of its first and last frames against the oracle (int8: the FP64
+    direct NCC; fft: the restatement of the reference algorithm, near-ties classified on FP64 surfaces)"""
+    specs = synthetic.clip_specs()
+    if method == 'fft':
+        specs = [s for s in specs if s['window'] + 2 * s['search'] <= 192]
+    vids = videos(specs)
+    batch, out = run_batch(specs, vids, method, tune=method == 'int8')
+    rng = np.random.default_rng(17)
+    for c, (s, (ref, fr)) in enumerate(zip(specs, vids)):
+        where = f'clip {c}: {s["height"]}p W{s["window"]} R{s["search"]} x{s["frames"]}'
+        expect = solo(s, ref, fr, method, batch.plan(c)['variant'])
+        same(out[c], expect, where)
+        assert np.isfinite(expect['offsets'][..., 0]).mean() > 0.5, where
+        o = OracleCorrelator(s['height'], s['width'], s['down'], s['across'], window=s['window'], search=s['search'])
+        o.set_reference(ref)
+        fsel = [0, s['frames'] - 1]
+        host = fr[fsel].cpu().numpy()
+        nwin = len(s['down']) * len(s['across'])
+        sample = np.sort(rng.choice(nwin, min(6, nwin), replace=False))
+        got = flat_select({k: v[fsel] for k, v in expect.items()}, sample)
+        if method == 'int8':
+            assert_int8_parity(got, flat_select(o.correlate_direct(host, windows=sample), sample))
+        else:
+            want = o.correlate_fft(host, windows=sample)
+            surf = o.correlate_direct(host, windows=sample, surfaces=True)['surface']
+            keep = np.zeros(want['peak'].shape[1:], bool)
+            keep.flat[sample] = True
+            g2 = {k: v[fsel].copy() for k, v in expect.items()}
+            for k in g2:   # the windows the oracle skipped: its fill values
+                g2[k][:, ~keep] = want[k][:, ~keep]
+            assert_fft_parity(g2, want, s['search'], surf=surf)

@@ -146,3 +146,103 @@ def test_gauss_newton_beside_quadratic_fit(gpu):
     assert np.median(err_gn) < 0.05
     assert np.median(err_gn) <= np.median(err_quad) + 0.005
     assert np.all(ncc.cpu().numpy()[ok] > 0.5)
+
+
+# ---- the differentiation workload at its own size: 1080 x 1920, window 64, search 32, skip 32 (1624
+# windows), 128 frames made on the GPU
+
+@pytest.fixture(scope='module')
+def full():
+    d = synthetic.smooth_shifts_device(n=128, device='cuda')
+    down, across = framecor.window_grid(1080, 1920, 64, 32, 32)
+    cor = framecor.Correlator(1080, 1920, down, across, window=64, search=32, method='int8')
+    cor.set_reference(d['ref'])
+    return d, down, across, cor
+
+
+def ncc_reference_windows(frames, ref, down, across, W, gross, shifts, wins):
+    """ncc_reference restricted to the windows wins ((row, column) pairs): frames (n, h, w) float64,
+    shifts (n, len(wins), 2) float64, gross (n, ny, nx, 2); returns (n, len(wins))"""
+    out = []
+    for i, (a, b) in enumerate(wins):
+        g = gross[:, a:a + 1, b:b + 1]
+        out.append(ncc_reference(frames, ref, down[a:a + 1], across[b:b + 1], W, g, shifts[:, i:i + 1, None])[:, 0, 0])
+    return torch.stack(out, 1)
+
+
+def test_full_size_gradients_on_sampled_windows(gpu, full):
+    """The 1080p geometry: forward, backward (frame and shift gradients) and central differences on 16
+    sampled windows of 2 frames, against the FP64 torch restatement; the loss weights only those windows,
+    so the frame gradient of the GPU (every window's contribution, atomically added) must equal theirs"""
+    d, down, across, cor = full
+    ny, nx = len(down), len(across)
+    rng = np.random.default_rng(12)
+    fsel = [5, 77]
+    frames = d['frames'][fsel].to(torch.float32).requires_grad_(True)
+    flat = rng.choice(ny * nx, 16, replace=False)
+    wins = [(int(k // nx), int(k % nx)) for k in flat]
+    # away from the integer kinks of the bilinear sampling (central differences across one fail)
+    shifts = rng.choice([-1.0, 1.0], size=(2, ny, nx, 2)) * rng.uniform(0.05, 0.9, size=(2, ny, nx, 2))
+    gross = np.broadcast_to(d['gross'][fsel][:, None, None, :], shifts.shape).astype(np.int32)
+    weights = np.zeros((2, ny, nx))
+    for a, b in wins:
+        weights[:, a, b] = rng.normal(size=2)
+    s = torch.tensor(shifts, dtype=torch.float32, device='cuda', requires_grad=True)
+    ncc = autograd.ncc_shift(cor, frames, s, gross=gross)
+    (ncc * torch.tensor(weights, dtype=torch.float32, device='cuda')).sum().backward()
+    # the FP64 restatement on the sampled windows only
+    f64 = frames.detach().cpu().double().requires_grad_(True)
+    s64 = torch.tensor(s.detach().cpu().numpy()[:, [a for a, _ in wins], [b for _, b in wins]], dtype=torch.float64,
+                       requires_grad=True)
+    w_sel = torch.tensor(np.stack([weights[:, a, b] for a, b in wins], 1))
+    ref_ncc = ncc_reference_windows(f64, d['ref'], down, across, 64, gross, s64, wins)
+    (ref_ncc * w_sel).sum().backward()
+    got = ncc.detach().cpu().numpy()[:, [a for a, _ in wins], [b for _, b in wins]]
+    assert np.abs(got - ref_ncc.detach().numpy()).max() < 2e-6
+    gs = s.grad.cpu().numpy()[:, [a for a, _ in wins], [b for _, b in wins]]
+    assert np.abs(gs - s64.grad.numpy()).max() < 1e-4 * max(1.0, np.abs(s64.grad.numpy()).max())
+    gf, gf64 = frames.grad.cpu().numpy(), f64.grad.numpy()
+    assert np.abs(gf - gf64).max() < 1e-4 * np.abs(gf64).max()
+    # windows with zero weight have zero shift gradients
+    rest = np.ones((ny, nx), bool)
+    for a, b in wins:
+        rest[a, b] = False
+    assert np.all(s.grad.cpu().numpy()[:, rest] == 0)
+    # central differences (h = 1e-3) of the FP64 restatement
+    h = 1e-3
+    base = torch.tensor(s.detach().cpu().numpy()[:, [a for a, _ in wins], [b for _, b in wins]], dtype=torch.float64)
+    f64d = frames.detach().cpu().double()
+    for axis in range(2):
+        e = torch.zeros_like(base)
+        e[..., axis] = h
+        fd = ((ncc_reference_windows(f64d, d['ref'], down, across, 64, gross, base + e, wins)
+               - ncc_reference_windows(f64d, d['ref'], down, across, 64, gross, base - e, wins)) / (2 * h)).numpy()
+        # the GPU's gradient of sum(w ncc) is w d ncc / d shift
+        fdw = fd * w_sel.numpy()
+        assert np.abs(fdw - gs[..., axis]).max() < 2e-3 * max(1.0, np.abs(fdw).max())
+
+
+def test_full_size_gauss_newton_beside_quadratic_fit(gpu, full):
+    """All 128 frames of the 1080p workload: the integer peak and quadratic fit of the int8 correlator,
+    then three Gauss-Newton steps, against the known sub-pixel shifts (the mean of the displacement field
+    over each window)"""
+    d, down, across, cor = full
+    gross = d['gross']
+    r = cor.correlate(d['frames'], gross=gross)
+    quad = r['offsets'].cpu().numpy().astype(np.float64)
+    valid = np.isfinite(quad[..., 0])
+    assert valid.mean() > 0.9
+    g4 = np.broadcast_to(gross[:, None, None, :], quad.shape)
+    start = np.where(np.isfinite(quad), quad - g4, 0.0)
+    frames = d['frames'].to(torch.float32)
+    refined, ncc = autograd.refine(cor, frames, torch.tensor(start, dtype=torch.float32, device='cuda'),
+                                   gross=gross, steps=3)
+    gn = refined.cpu().numpy() + g4
+    ok = valid & np.isfinite(gn[..., 0])
+    assert ok.mean() > 0.9
+    err_quad = np.abs(quad - d['truth'])[ok]
+    err_gn = np.abs(gn - d['truth'])[ok]
+    print(f'1080p, 128 frames, {int(ok.sum())} window-frames: median |error| quadratic fit '
+          f'{np.median(err_quad):.4f} px, Gauss-Newton {np.median(err_gn):.4f} px')
+    assert np.median(err_gn) < 0.05
+    assert np.median(err_gn) <= np.median(err_quad) + 0.005
