@@ -1,5 +1,6 @@
 """
-Build of libframecor_hip.so: hipcc for gfx950, in tree (slipcam_amd/libframecor_hip.so).
+Build of libframecor_hip.so (hipcc for gfx950) and of the pybind11 module framecor._framecor on its C
+ABI (g++), in tree: slipcam_amd/libframecor_hip.so, slipcam_amd/framecor/_framecor*.so.
 
   python -m slipcam_amd.build          # or __graft_entry__.build()
 """
@@ -9,12 +10,15 @@ import os
 import shutil
 import subprocess
 import sys
+import sysconfig
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 BUILD = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'libframecor_hip.so')
+MODULE_SRC = 'framecor_module.cpp'
+MODULE = os.path.join(HERE, 'framecor', '_framecor' + sysconfig.get_config_var('EXT_SUFFIX'))
 SOURCES = ['capi.hip', 'batch.hip', 'sat.hip', 'int8_corr.hip', 'fft_corr.hip', 'diff.hip']
 VERSION = '0.1.0'
 
@@ -53,15 +57,36 @@ def build(verbose=True):
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as pool:
         objs = list(pool.map(_compile, SOURCES))
     tmp = LIB + '.tmp'
-    cmd = [hipcc(), '--offload-arch=gfx950', '-shared', '-fPIC', '-o', tmp] + objs + ['-lpthread']
+    cmd = [hipcc(), '--offload-arch=gfx950', '-shared', '-fPIC', '-Wl,-soname,libframecor_hip.so', '-o', tmp] + objs \
+        + ['-lpthread']
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode:
         raise RuntimeError(f'link failed:\n{r.stderr}')
     os.replace(tmp, LIB)
     if verbose:
         print(f'built {LIB}')
+    build_module(verbose)
     return LIB
 
 
+def build_module(verbose=True):
+    """The pybind11 module _framecor (host C++, g++) linked to libframecor_hip.so beside the package"""
+    import pybind11
+    tmp = MODULE + '.tmp'
+    cmd = ['g++', '-O2', '-std=c++17', '-shared', '-fPIC', '-fvisibility=hidden', '-Wall',
+           '-I', sysconfig.get_paths()['include'], '-I', pybind11.get_include(), '-I', os.path.join(ROOT, 'include'),
+           os.path.join(CSRC, MODULE_SRC), '-L', HERE, '-l:libframecor_hip.so', "-Wl,-rpath,$ORIGIN/..", '-o', tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode:
+        raise RuntimeError(f'g++ failed on {MODULE_SRC}:\n{r.stderr}')
+    os.replace(tmp, MODULE)
+    if verbose:
+        print(f'built {MODULE}')
+    return MODULE
+
+
 if __name__ == '__main__':
-    build()
+    if sys.argv[1:] == ['--module']:
+        build_module()
+    else:
+        build()

@@ -12,6 +12,11 @@ The API of the reference's slipcam.framecor (python/slipcam/framecor/__init__.py
 plus the keyword method='fft' (the reference's FP32 FFT algorithm, the default) or 'int8' (exact
 integer correlation on the int8 matrix cores, FP64 peak fit), and device-resident inputs: correlate()
 also takes a torch uint8 tensor on the GPU and then returns torch tensors without a host copy.
+
+Two bindings of one library (libframecor_hip.so, its C ABI in include/framecor_hip.h): the pybind11
+module _framecor, the reference's extension module (Correlator, has_cuda, __version__), which
+Correlator here extends; and ctypes for the entry points beyond the reference's API (device-resident
+calls, NCC surfaces, batches, the differentiable form).
 """
 
 import ctypes
@@ -19,11 +24,10 @@ import os
 
 import numpy as np
 
-__all__ = ['Batch', 'Correlator', 'has_gpu', 'window_grid', '__version__']
+__all__ = ['Batch', 'Correlator', 'has_cuda', 'has_gpu', 'window_grid', '__version__']
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-# FRAMECOR_LIBRARY: another build of the library (for example one compiled with other options)
-_LIB_PATH = os.environ.get('FRAMECOR_LIBRARY') or os.path.join(_PKG, 'libframecor_hip.so')
+_LIB_PATH = os.path.join(_PKG, 'libframecor_hip.so')
 
 _OK, _EINVAL, _ERUNTIME, _ENOMEM = 0, 1, 2, 3
 _METHODS = {'fft': 0, 'int8': 1}
@@ -105,7 +109,9 @@ def _load():
 
 
 _lib = _load()
-__version__ = _lib.framecor_version().decode()
+# the reference's module API (cxx/bindings.cpp:111-142): pybind11 on the same library
+from ._framecor import __version__, has_cuda  # noqa: E402
+from . import _framecor  # noqa: E402
 
 
 def _raise(status):
@@ -153,14 +159,17 @@ def _is_device_tensor(x):
     return t.__module__.startswith('torch') and t.__name__ == 'Tensor' and x.is_cuda
 
 
-class Correlator:
+class Correlator(_framecor.Correlator):
     """
-    The correlator of 8-bit frames with a reference frame, on a grid of square windows
+    The correlator of 8-bit frames with a reference frame, on a grid of square windows: the pybind11
+    class of the reference's API (_framecor.Correlator: set_reference, correlate of numpy frames with
+    the GIL released, backend, shape, window, search) plus the device-resident and extra entry points
 
     height, width: the size of the frames
     down, across: the first pixels of the rows and of the columns of windows
     window: the size of the windows; search: the half search range, pixels
-    backend: 'hip' or 'auto' (the GPU); the reference's 'cuda' and 'cpu' are not backends here
+    backend: 'hip', 'auto' or 'cuda' (the reference's name of its GPU backend): the GPU; the
+    reference's 'cpu' is not a backend here (no CPU path: ValueError)
     method: 'fft' (the reference's FP32 FFT algorithm) or 'int8' (exact int8 matrix-core correlation)
     batch: at least 1 (kept for the reference's API); stat_size: the SNR window
     device: the GPU; threads: kept for the reference's API (the CPU threads of its CPU backend)
@@ -168,69 +177,23 @@ class Correlator:
 
     def __init__(self, height, width, down, across, *, window=64, search=48, backend='auto', batch=4096,
                  stat_size=21, device=0, threads=0, method='fft'):
-        self._c = None
-        down = np.ascontiguousarray(down, np.int32)
-        across = np.ascontiguousarray(across, np.int32)
+        down = np.asarray(down)
+        across = np.asarray(across)
         if down.ndim != 1 or across.ndim != 1:
             raise ValueError('framecor: down and across must be sequences of integers')
-        m = _METHODS.get(method)
-        # the reference checks the parameters first, then the backend (cxx/bindings.cpp:37-44)
-        _check(_lib.framecor_check_params(int(height), int(width), _ptr(down), len(down), _ptr(across), len(across),
-                                          int(window), int(search), 0 if m is None else m, int(batch), int(stat_size)))
-        if backend not in ('auto', 'hip'):
-            raise ValueError(f"framecor: the backend must be auto or hip, not {backend}")
-        if m is None:
-            raise ValueError(f"framecor: the method must be fft or int8, not {method}")
-        int(threads)
-        handle = ctypes.c_void_p()
-        _check(_lib.framecor_create(int(height), int(width), _ptr(down), len(down), _ptr(across), len(across),
-                                    int(window), int(search), m, int(batch), int(stat_size), int(device),
-                                    ctypes.byref(handle)))
-        self._c = handle
-        self._destroy = _lib.framecor_destroy
+        super().__init__(int(height), int(width), [int(v) for v in down], [int(v) for v in across],
+                         window=int(window), search=int(search), backend=backend, batch=int(batch),
+                         stat_size=int(stat_size), device=int(device), threads=int(threads), method=method)
+        # the C handle for the entry points beyond the reference's API (owned by the base class)
+        self._c = ctypes.c_void_p(self._handle)
         self._h, self._w = int(height), int(width)
-        self._window, self._search = int(window), int(search)
-        self._shape = (len(down), len(across))
-        self._method = method
         self._device = int(device)
-        self._ready = False
-
-    def __del__(self):
-        c, destroy = getattr(self, '_c', None), getattr(self, '_destroy', None)
-        if c and destroy is not None:
-            destroy(c)
-            self._c = None
-
-    @property
-    def backend(self):
-        return _lib.framecor_backend(self._c).decode()
-
-    @property
-    def shape(self):
-        """the number of windows (down, across)"""
-        return self._shape
-
-    @property
-    def window(self):
-        return self._window
-
-    @property
-    def search(self):
-        return self._search
-
-    @property
-    def method(self):
-        return self._method
 
     def set_reference(self, frame):
-        """Set the reference frame (height x width, uint8)"""
+        """Set the reference frame (height x width, uint8; numpy, or a torch tensor)"""
         if _is_device_tensor(frame):
             frame = frame.cpu().numpy()
-        frame = np.ascontiguousarray(frame, np.uint8)
-        if frame.ndim != 2 or frame.shape != (self._h, self._w):
-            raise ValueError('framecor: the reference frame must be height x width')
-        _check(_lib.framecor_set_reference(self._c, _ptr(frame)))
-        self._ready = True
+        super().set_reference(frame)
 
     def set_variant(self, variant):
         """The int8 kernel: 'auto', 'simple' (one frame at a time), 'piped' (producer/helper pipeline) or
@@ -251,7 +214,7 @@ class Correlator:
         return dict(box_sums=ms[0], correlate=ms[1], launches=launches.value)
 
     def _gross(self, gross, n, module):
-        ny, nx = self._shape
+        ny, nx = self.shape
         g = gross
         if g.ndim == 2 and g.shape[0] == n and g.shape[1] == 2:
             return module.broadcast(g, n, ny, nx)
@@ -275,20 +238,7 @@ class Correlator:
             raise RuntimeError('framecor: set_reference first')
         if _is_device_tensor(frames):
             return self._correlate_device(frames, gross)
-        frames = np.ascontiguousarray(frames, np.uint8)
-        if frames.ndim != 3 or frames.shape[1] != self._h or frames.shape[2] != self._w:
-            raise ValueError('framecor: the frames must be frames x height x width')
-        n = frames.shape[0]
-        ny, nx = self._shape
-        g = None
-        if gross is not None:
-            g = np.ascontiguousarray(self._gross(np.asarray(gross), n, _NumpyOps), np.int32)
-        out = dict(offsets=np.empty((n, ny, nx, 2), np.float32), peak=np.empty((n, ny, nx), np.float32),
-                   snr=np.empty((n, ny, nx), np.float32), cov=np.empty((n, ny, nx, 3), np.float32))
-        if n > 0:
-            _check(_lib.framecor_correlate(self._c, _ptr(frames), n, _ptr(g), _ptr(out['offsets']),
-                                           _ptr(out['peak']), _ptr(out['snr']), _ptr(out['cov'])))
-        return out
+        return super().correlate(frames, gross)
 
     def _correlate_device(self, frames, gross):
         import torch
@@ -298,7 +248,7 @@ class Correlator:
             raise ValueError(f'framecor: the frames are on {frames.device}, the correlator on GPU {self._device}')
         frames = frames.contiguous()
         n = frames.shape[0]
-        ny, nx = self._shape
+        ny, nx = self.shape
         g = None
         if gross is not None:
             if not isinstance(gross, torch.Tensor):
@@ -330,8 +280,8 @@ class Correlator:
         if frame.device.index != self._device:
             raise ValueError(f'framecor: the frame is on {frame.device}, the correlator on GPU {self._device}')
         frame = frame.contiguous()
-        ny, nx = self._shape
-        M = 2 * self._search + 1
+        ny, nx = self.shape
+        M = 2 * self.search + 1
         out = torch.empty((ny, nx, M, M), dtype=torch.float64, device=frame.device)
         stream = torch.cuda.current_stream(frame.device).cuda_stream
         _check(_lib.framecor_ncc_surfaces_device(self._c, frame.data_ptr(), None, 0, ny * nx, out.data_ptr(), stream))
@@ -351,7 +301,7 @@ class Correlator:
         frame = np.ascontiguousarray(frame, np.uint8)
         if frame.ndim != 2 or frame.shape != (self._h, self._w):
             raise ValueError('framecor: the frame must be height x width')
-        ny, nx = self._shape
+        ny, nx = self.shape
         g = None
         if gross is not None:
             g = np.asarray(gross)
@@ -374,7 +324,7 @@ class Correlator:
         frame = np.ascontiguousarray(frame, np.uint8)
         if frame.ndim != 2 or frame.shape != (self._h, self._w):
             raise ValueError('framecor: the frame must be height x width')
-        out = np.empty(self._shape, np.float64)
+        out = np.empty(self.shape, np.float64)
         _check(_lib.framecor_periodicity(self._c, _ptr(frame), int(exclude), _ptr(out)))
         return out
 
@@ -537,12 +487,6 @@ class Batch:
             torch.cuda.current_stream(dev).synchronize()
             return [{k: v.cpu().numpy() for k, v in o.items()} for o in outs]
         return outs
-
-
-class _NumpyOps:
-    @staticmethod
-    def broadcast(g, n, ny, nx):
-        return np.broadcast_to(g.astype(np.int32)[:, None, None, :], (n, ny, nx, 2))
 
 
 class _TorchOps:

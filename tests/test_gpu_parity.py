@@ -281,3 +281,26 @@ def test_int8_runs_of_frames_agree(gpu, n):
     assert np.isfinite(whole['offsets'][..., 0]).mean() > 0.9
     for key in whole:
         assert np.array_equal(whole[key], np.concatenate([o[key] for o in one]), equal_nan=True), key
+
+
+@pytest.mark.parametrize('method', ['int8', 'fft'])
+def test_set_reference_waits_for_device_calls(gpu, fault, method):
+    """Device calls queued on another stream, then set_reference of another frame at once: the queued
+    calls still see the first reference (the correlator orders its own stream after them)"""
+    torch = _torch()
+    d, down, across, _ = fault
+    cor = framecor.Correlator(720, 1280, down, across, window=64, search=48, method=method)
+    cor.set_reference(d['ref'])
+    dev = torch.from_numpy(d['frames']).cuda()
+    expect = {k: v.cpu().numpy() for k, v in cor.correlate(dev).items()}
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        outs = [cor.correlate(dev) for _ in range(8)]
+        cor.set_reference(np.roll(d['ref'], 7, axis=1))
+    torch.cuda.synchronize()
+    for out in outs:
+        for k in expect:
+            assert np.array_equal(out[k].cpu().numpy(), expect[k], equal_nan=True), k
+    moved = cor.correlate(dev)
+    assert not np.array_equal(moved['offsets'].cpu().numpy(), expect['offsets'], equal_nan=True)

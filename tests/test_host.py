@@ -70,9 +70,13 @@ def test_arguments_without_gpu():
         framecor.Correlator(240, 320, [0], [300], window=32, search=8, backend='hip')
     with pytest.raises(ValueError, match='outside the frame'):
         framecor.Correlator(240, 320, [0], [300], window=32, search=8, backend='gpu')   # parameters first
-    for backend in ('gpu', 'cuda', 'cpu'):
+    for backend in ('gpu', 'cpu', 'CUDA'):
         with pytest.raises(ValueError, match='backend'):
             framecor.Correlator(240, 320, [0], [0], window=32, search=8, backend=backend)
+    # 'cuda' (the reference's name of its GPU backend) is accepted: parameters are still checked first
+    with pytest.raises(ValueError, match='outside the frame'):
+        framecor.Correlator(240, 320, [0], [300], window=32, search=8, backend='cuda')
+    assert framecor.has_cuda() == framecor.has_gpu()
     with pytest.raises(ValueError, match='method'):
         framecor.Correlator(240, 320, [0], [0], window=32, search=8, method='fp16')
     with pytest.raises(ValueError, match='at least 3 pixels'):
@@ -179,3 +183,30 @@ def test_header_is_plain_c_and_a_c_program_calls_the_library(tmp_path):
     version, ok, bad, msg = out.split('|', 3)
     assert version == framecor.__version__ and int(ok) == 0 and int(bad) == 1
     assert msg == 'framecor: the window must be at least 3 pixels'
+
+
+def test_framecor_module_without_gpu():
+    """The pybind11 module of the reference's API (cxx/bindings.cpp:111-142): its names, keywords and
+    defaults, and its argument errors before any GPU call, with the reference's messages"""
+    from slipcam_amd.framecor import _framecor
+    from slipcam_amd.framecor._framecor import Correlator, __version__, has_cuda  # noqa: F401
+    assert __version__ == framecor.__version__ == '0.1.0'
+    assert has_cuda() == framecor.has_gpu()
+    doc = re.sub(r'typing\.SupportsInt \| typing\.SupportsIndex', 'int', Correlator.__init__.__doc__)
+    assert ', *, window' in doc
+    for kw in ("window: int = 64", "search: int = 48", "backend: str = 'auto'", "batch: int = 4096",
+               "stat_size: int = 21", "device: int = 0", "threads: int = 0", "method: str = 'fft'"):
+        assert kw in doc, kw
+    with pytest.raises(ValueError, match=r'^framecor: a window is outside the frame \(across 300\)$'):
+        Correlator(240, 320, [0], [300], window=32, search=8, backend='cpu')
+    with pytest.raises(ValueError, match='^framecor: the backend must be auto, hip or cuda, not gpu$'):
+        Correlator(240, 320, [0], [0], window=32, search=8, backend='gpu')
+    with pytest.raises(ValueError, match='^framecor: the window must be at least 3 pixels$'):
+        Correlator(240, 320, [0], [0], window=2, search=8)
+    with pytest.raises(ValueError, match='SNR window'):
+        Correlator(240, 320, [0], [0], window=32, search=8, stat_size=2)
+    with pytest.raises(ValueError, match='method'):
+        Correlator(240, 320, [0], [0], window=32, search=8, method='fp16')
+    with pytest.raises(TypeError):
+        Correlator(240, 320, [0], [0], 32, 8)    # window and search are keyword-only
+    assert issubclass(framecor.Correlator, _framecor.Correlator)
