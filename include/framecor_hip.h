@@ -97,6 +97,19 @@ int framecor_correlate(framecor_correlator *c, const uint8_t *frames, int n, con
 int framecor_correlate_device(framecor_correlator *c, const uint8_t *frames, int n, const int32_t *gross,
                               float *offset, float *peak, float *snr, float *cov, void *stream);
 
+/* framecor_correlate_device with no device allocation. No counterpart in the reference: for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The box-sum maps of each group of frames go to a static 64 MiB device array per GPU,
+ * held until the process ends, instead of buffers the correlator allocates; frames are correlated in
+ * groups whose maps fit in it (12 bytes per window position and frame for the int8 method, 8 for the
+ * FFT: 64 MiB holds 6 frames at 720 x 1280, window 64). FRAMECOR_EINVAL if one frame's maps exceed
+ * 64 MiB. Every call on a GPU shares that GPU's array: calls are ordered on the GPU (each waits for the
+ * previous one's work on it), except under stream capture, where captured work that overlaps other
+ * calls on the same GPU races. The correlator must have been created and given its reference (those
+ * calls allocate). Arguments and results as framecor_correlate_device. */
+int framecor_correlate_device_noalloc(framecor_correlator *c, const uint8_t *frames, int n, const int32_t *gross,
+                                      float *offset, float *peak, float *snr, float *cov, void *stream);
+
 /* The differentiable form (no counterpart in the reference). For n float32 frames (device memory),
  * the windows of the correlator's reference and a continuous shift per window, shifts n x nwin x 2
  * (down, across, relative to the window plus its gross offset; gross as in framecor_correlate, or

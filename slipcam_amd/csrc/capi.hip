@@ -40,6 +40,7 @@ int framecor_create(int height, int width, const int *down, int ndown, const int
     return guarded([&] {
         auto c = std::make_unique<framecor_correlator>();
         c->init(height, width, down, ndown, across, nacross, window, search, method, batch, stat_size, device);
+        scratchSetup(device);
         *out = c.release();
     });
 }

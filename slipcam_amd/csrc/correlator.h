@@ -156,6 +156,10 @@ inline void checkAll(int height, int width, const int *down, int ndown, const in
     }
 }
 
+// the device address of the no-allocation entry point's scratch array on device (the current
+// device), resolved once per device (noalloc.hip); called when a correlator is created
+void scratchSetup(int device);
+
 } // namespace framecor
 
 using namespace framecor;
@@ -327,21 +331,29 @@ struct framecor_correlator {
         FC_HIP_CHECK(hipEventRecord(events[eventsUsed++], s));
     }
 
-    // one group of frames, all pointers on the device
+    // one group of frames, all pointers on the device, with the correlator's box-map buffers
     void runGroup(const uint8_t *frames, int n, const int32_t *gross, float *offset, float *peak, float *snr,
                   float *cov, hipStream_t s)
     {
+        runGroupWith(frames, n, gross, offset, peak, snr, cov, s, dBox.as<int32_t>(), dBox2.as<int32_t>(),
+                     dScreen.as<int2>(), true);
+    }
+
+    // the same with given box-map buffers (int8: screen and box2; FFT: box and box2, n x boxSize each);
+    // timed: the per-stage events of set_timing
+    void runGroupWith(const uint8_t *frames, int n, const int32_t *gross, float *offset, float *peak, float *snr,
+                      float *cov, hipStream_t s, int32_t *box, int32_t *box2, int2 *screen, bool timed)
+    {
         const bool isInt8 = method == FRAMECOR_METHOD_INT8;
-        mark(s);
-        launchBoxSums(frames, n, g.h, g.w, g.W, isInt8 ? nullptr : dBox.as<int32_t>(), dBox2.as<int32_t>(),
-                      isInt8 ? dScreen.as<int2>() : nullptr, s);
-        mark(s);
+        if (timed) mark(s);
+        launchBoxSums(frames, n, g.h, g.w, g.W, isInt8 ? nullptr : box, box2, isInt8 ? screen : nullptr, s);
+        if (timed) mark(s);
         FrameGroup fg;
         fg.frames = frames;
         fg.gross = gross;
-        fg.box = isInt8 ? nullptr : dBox.as<int32_t>();
-        fg.box2 = dBox2.as<int32_t>();
-        fg.screen = isInt8 ? dScreen.as<int2>() : nullptr;
+        fg.box = isInt8 ? nullptr : box;
+        fg.box2 = box2;
+        fg.screen = isInt8 ? screen : nullptr;
         fg.n = n;
         fg.offset = offset;
         fg.peak = peak;
@@ -368,7 +380,7 @@ struct framecor_correlator {
             a.rs = 2 * a.H;
             launchFftCorrelate(a, s);
         }
-        mark(s);
+        if (timed) mark(s);
     }
 
     void correlateDevice(const uint8_t *frames, int n, const int32_t *gross, float *offset, float *peak,
@@ -386,6 +398,43 @@ struct framecor_correlator {
             const size_t q0 = size_t(f0) * g.nwin;
             runGroup(frames + size_t(f0) * g.pixels, nf, gross ? gross + 2 * q0 : nullptr, offset + 2 * q0,
                      peak + q0, snr + q0, cov + 3 * q0, s);
+        }
+        done(s);
+    }
+
+    // correlateDevice with the box maps of each group in a caller's device buffer (scratch, bytes long,
+    // 256-byte aligned) instead of the correlator's own: no device allocation and no timing events;
+    // groups of as many frames as the buffer holds. A frame whose maps exceed the buffer: EINVAL.
+    void correlateScratch(const uint8_t *frames, int n, const int32_t *gross, float *offset, float *peak, float *snr,
+                          float *cov, hipStream_t s, void *scratch, size_t bytes)
+    {
+        if (!ready) throw std::runtime_error("framecor: set_reference first");
+        if (n < 0) invalid("the number of frames must not be negative");
+        const size_t perFrame = bytesPerFrame();
+        if (perFrame > bytes)
+            invalid("the box-sum maps of one frame (" + std::to_string(perFrame) + " bytes) exceed the "
+                    + std::to_string(bytes >> 20) + " MiB scratch of the no-allocation entry point");
+        if (n == 0) return;
+        FC_HIP_CHECK(hipSetDevice(device));
+        const int G = int(std::min<size_t>(size_t(maxGroup), bytes / perFrame));
+        const size_t plane = size_t(G) * g.boxSize;
+        unsigned char *base = static_cast<unsigned char *>(scratch);
+        int32_t *box = nullptr, *box2 = nullptr;
+        int2 *screen = nullptr;
+        if (method == FRAMECOR_METHOD_INT8) {
+            screen = reinterpret_cast<int2 *>(base);
+            box2 = reinterpret_cast<int32_t *>(base + plane * sizeof(int2));
+        }
+        else {
+            box = reinterpret_cast<int32_t *>(base);
+            box2 = reinterpret_cast<int32_t *>(base + plane * sizeof(int32_t));
+        }
+        after(s);
+        for (int f0 = 0; f0 < n; f0 += G) {
+            const int nf = std::min(G, n - f0);
+            const size_t q0 = size_t(f0) * g.nwin;
+            runGroupWith(frames + size_t(f0) * g.pixels, nf, gross ? gross + 2 * q0 : nullptr, offset + 2 * q0,
+                         peak + q0, snr + q0, cov + 3 * q0, s, box, box2, screen, false);
         }
         done(s);
     }

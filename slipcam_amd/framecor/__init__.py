@@ -63,6 +63,8 @@ def _load():
     lib.framecor_correlate.argtypes = [P, P, I, P, P, P, P, P]
     lib.framecor_correlate_device.restype = I
     lib.framecor_correlate_device.argtypes = [P, P, I, P, P, P, P, P, P]
+    lib.framecor_correlate_device_noalloc.restype = I
+    lib.framecor_correlate_device_noalloc.argtypes = [P, P, I, P, P, P, P, P, P]
     lib.framecor_ncc_forward.restype = I
     lib.framecor_ncc_forward.argtypes = [P, P, I, P, P, P, P]
     lib.framecor_ncc_backward.restype = I
