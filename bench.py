@@ -133,7 +133,7 @@ def time_batch(torch, framecor, specs, vids, method, steps, warmup):
 def batch_workload(torch, framecor, synthetic, steps, warmup):
     """The batching workload: 16 seeded clips of uneven sizes (resolutions 480x640 / 720x1280 /
     1080x1920, windows 32/64/128, search 16/32/64, 8-256 frames, skip = window / 4), frames made on the
-    GPU; int8 over all clips, fft over those whose search window fits its LDS (L <= 192)"""
+    GPU; both forms over all clips"""
     specs = synthetic.clip_specs()
     vids = [synthetic.clip_video(s, 'cuda')[:2] for s in specs]
     torch.cuda.synchronize()
@@ -141,10 +141,8 @@ def batch_workload(torch, framecor, synthetic, steps, warmup):
                                                 for s in specs)}
     log('bench: batch int8')
     out['int8'] = time_batch(torch, framecor, specs, vids, 'int8', steps, warmup)
-    keep = [i for i, s in enumerate(specs) if s['window'] + 2 * s['search'] <= 192]
     log('bench: batch fft')
-    out['fft'] = time_batch(torch, framecor, [specs[i] for i in keep], [vids[i] for i in keep], 'fft', steps, warmup)
-    out['fft']['clips_note'] = f'{len(keep)} of {len(specs)} clips (search window <= 192)'
+    out['fft'] = time_batch(torch, framecor, specs, vids, 'fft', steps, warmup)
     return out
 
 

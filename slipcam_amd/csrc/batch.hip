@@ -42,6 +42,9 @@ struct Shape {
     int fpb = 0;        // frames per item; 0: automatic
     int perCu = 0;      // resident blocks per CU; 0: as many as fit
     bool tuned = false;
+    // the FFT form with spectra beyond LDS: the per-block workspace of at most workBlocks resident blocks
+    std::shared_ptr<DeviceBuffer> work;
+    int workBlocks = 0;
     // the launch of the plan (prepared before a capture: no allocation or query inside one)
     const void *func = nullptr;
     int threads = 0;
@@ -123,7 +126,10 @@ struct framecor_batch {
         int occ = 0;
         FC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, sh.func, sh.threads, sh.shmem));
         occ = std::max(occ, 1);
+        // spectra in a workspace: at most its blocks resident
+        if (sh.work) occ = std::max(1, std::min(occ, sh.workBlocks / std::max(cus, 1)));
         sh.blocks = cus * (sh.perCu > 0 ? std::min(sh.perCu, occ) : occ);
+        if (sh.work) sh.blocks = std::min(sh.blocks, sh.workBlocks);
         return sh.perCu <= occ;
     }
 
@@ -179,6 +185,11 @@ struct framecor_batch {
         a.fg = fg;
         a.spectra = k.dSpectra.as<float2>();
         a.twiddle = k.dTwiddle.as<float2>();
+        // spectra beyond LDS: the shape's workspace, one per resident block
+        const Shape &sh = shapes[shapeOf[c]];
+        a.work = sh.work ? sh.work->as<float2>() : nullptr;
+        a.workStride = fftWorkPerBlock(k.g.L) / sizeof(float2);
+        a.workBlocks = sh.workBlocks;
         a.plan = k.plan;
         a.np = (k.g.L + 1) / 2;
         a.H = k.g.L / 2 + 1;
@@ -487,6 +498,11 @@ int framecor_batch_create(const framecor_clip *clips, int nclips, int method, in
                 sh.L = g.L;
                 sh.snr = std::min(g.stat, g.M);
                 if (b->isInt8()) sh.layout = b->clips[c]->i8;
+                else if (!fftInLds(g.L)) {
+                    sh.workBlocks = fftWorkBlocks(g.L, b->cus);
+                    sh.work = std::make_shared<DeviceBuffer>();
+                    sh.work->ensure(fftWorkPerBlock(g.L) * size_t(sh.workBlocks));
+                }
                 b->shapes.push_back(sh);
             }
             b->shapes[s].clips.push_back(c);

@@ -152,7 +152,7 @@ inline void checkAll(int height, int width, const int *down, int ndown, const in
         FftPlan plan;
         if (!fftPlanFor(g.L, plan))
             invalid("the fft method needs a search window (window + 2 search = " + std::to_string(g.L)
-                    + ") whose half spectrum fits in LDS (at most 192) and whose prime factors are at most 31");
+                    + ") of at most " + std::to_string(kFftMaxL) + " pixels");
     }
 }
 
@@ -180,6 +180,9 @@ struct framecor_correlator {
 
     // per window, written by set_reference
     DeviceBuffer dDown, dAcross, dNorm, dTsum, dTvar, dTmpl, dSpectra, dTwiddle;
+    // the FFT form's per-block spectra when they do not fit in LDS (workBlocks blocks)
+    DeviceBuffer dWork;
+    int workBlocks = 0;
     // per group of frames
     DeviceBuffer dBox, dBox2, dScreen;
     int groupCap = 0;
@@ -289,7 +292,21 @@ struct framecor_correlator {
             FC_HIP_CHECK(hipMemcpy(dTwiddle.ptr, tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
             const size_t np = (L + 1) / 2, rs = 2 * (L / 2 + 1);
             dSpectra.ensure(sizeof(float2) * np * rs * size_t(g.nwin));
+            if (!fftInLds(L)) {
+                int cus = 0;
+                FC_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+                workBlocks = fftWorkBlocks(L, cus);
+                dWork.ensure(fftWorkPerBlock(L) * size_t(workBlocks));
+            }
         }
+    }
+
+    // the global workspace of the FFT form's blocks, when the spectrum does not fit in LDS
+    void setWork(FftArgs &a) const
+    {
+        a.work = dWork.as<float2>();
+        a.workStride = fftWorkPerBlock(g.L) / sizeof(float2);
+        a.workBlocks = workBlocks;
     }
 
     WindowData windowData() const
@@ -374,6 +391,7 @@ struct framecor_correlator {
             a.fg = fg;
             a.spectra = dSpectra.as<float2>();
             a.twiddle = dTwiddle.as<float2>();
+            setWork(a);
             a.plan = plan;
             a.np = (g.L + 1) / 2;
             a.H = g.L / 2 + 1;
@@ -410,6 +428,9 @@ struct framecor_correlator {
     {
         if (!ready) throw std::runtime_error("framecor: set_reference first");
         if (n < 0) invalid("the number of frames must not be negative");
+        if (method == FRAMECOR_METHOD_FFT && !fftInLds(g.L))
+            invalid("the no-allocation entry point does not take FFT search windows whose spectrum does not fit in "
+                    "LDS (window + 2 search = " + std::to_string(g.L) + "): they need the correlator's workspace");
         const size_t perFrame = bytesPerFrame();
         if (perFrame > bytes)
             invalid("the box-sum maps of one frame (" + std::to_string(perFrame) + " bytes) exceed the "
@@ -688,6 +709,8 @@ struct framecor_correlator {
             a.twiddle = dTwiddle.as<float2>();
             a.templ = dTempl.as<float>();
             a.spectraOut = dSpectra.as<float2>();
+            // a spectrum that does not fit in LDS is transformed in place in the output
+            a.work = fftInLds(g.L) ? nullptr : dSpectra.as<float2>();
             a.plan = plan;
             a.np = (g.L + 1) / 2;
             a.H = g.L / 2 + 1;

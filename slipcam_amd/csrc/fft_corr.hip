@@ -11,6 +11,8 @@
 // covariance. HBM traffic per window and frame: the chip, the reference spectrum, the box sums of the
 // surface and the outputs; nothing else leaves LDS.
 
+#include <algorithm>
+
 #include "common.h"
 #include "epilogue.h"
 #include "fft_phases.h"
@@ -37,11 +39,14 @@ struct FftShared {
     int *sums;      // waves: the chip sum
 };
 
-__device__ inline FftShared carve(unsigned char *lds, const FftArgs &a)
+// the block's buffers: the half spectrum Z in LDS, or (zg not null) in the block's global workspace,
+// then the tables and reductions in LDS
+__device__ inline FftShared carve(unsigned char *lds, const FftArgs &a, float2 *zg)
 {
     FftShared s;
-    s.Z = reinterpret_cast<float2 *>(lds);
-    s.tw = s.Z + size_t(a.np) * a.rs;
+    float2 *z = reinterpret_cast<float2 *>(lds);
+    s.Z = zg ? zg : z;
+    s.tw = zg ? z : z + size_t(a.np) * a.rs;
     s.redd = reinterpret_cast<double *>(s.tw + a.g.L);
     s.pos = reinterpret_cast<int *>(s.redd + kFftThreads / kWave);
     s.redv = reinterpret_cast<float *>(s.pos + a.g.L);
@@ -69,12 +74,14 @@ __device__ inline void forward2d(const FftShared &s, const FftArgs &a, int tid)
     transformLines<false>(s.Z, s.tw, L, a.plan.fac, a.plan.nf, a.H, ColAddr{a.rs, a.H}, tid, kFftThreads, sync);
 }
 
-// the spectra of the reference windows: one block per window
+// the spectra of the reference windows: one block per window (a spectrum that does not fit in LDS is
+// transformed in place in the output)
 __global__ void __launch_bounds__(kFftThreads) fftReference(FftArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
-    const FftShared s = carve(lds, a);
     const int tid = threadIdx.x, k = blockIdx.x;
+    float2 *out = a.spectraOut + size_t(k) * a.np * a.rs;
+    const FftShared s = carve(lds, a, a.work ? out : nullptr);
     const int W = a.g.W;
     loadTables(s, a, tid);
     // zero-padded template, packed: row y -> half (y & 1) of pair y >> 1
@@ -88,15 +95,16 @@ __global__ void __launch_bounds__(kFftThreads) fftReference(FftArgs a)
     }
     __syncthreads();
     forward2d(s, a, tid);
-    float2 *out = a.spectraOut + size_t(k) * a.np * a.rs;
-    for (int i = tid; i < a.np * a.rs; i += kFftThreads) out[i] = s.Z[i];
+    if (!a.work)
+        for (int i = tid; i < a.np * a.rs; i += kFftThreads) out[i] = s.Z[i];
 }
 
-// window k against frames fBegin .. fEnd - 1, by the whole block
+// window k against frames fBegin .. fEnd - 1, by the whole block; zg: the block's global workspace for
+// the half spectrum, or null (in LDS)
 __device__ __attribute__((always_inline)) inline void fftWindow(const FftArgs &a, int k, int fBegin, int fEnd,
-                                                                unsigned char *lds)
+                                                                unsigned char *lds, float2 *zg)
 {
-    const FftShared s = carve(lds, a);
+    const FftShared s = carve(lds, a, zg);
     const Geometry &g = a.g;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
     const int L = g.L, M = g.M, W = g.W;
@@ -227,7 +235,22 @@ __global__ void __launch_bounds__(kFftThreads) fftCorrelate(FftArgs a)
     const int logical = xcdRemap(blockIdx.x, gridDim.x);
     const int k = logical / a.nfb, fblock = logical - k * a.nfb;
     const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n);
-    fftWindow(a, k, fBegin, fEnd, lds);
+    fftWindow(a, k, fBegin, fEnd, lds, nullptr);
+}
+
+// the same for search windows whose spectrum is in the global workspace: a persistent grid (one
+// workspace per block) over the (window, run of frames) items, in the order of the blocks above
+__global__ void __launch_bounds__(kFftThreads) fftCorrelateGlobal(FftArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    float2 *zg = a.work + size_t(blockIdx.x) * a.workStride;
+    const int items = a.g.nwin * a.nfb;
+    for (int it = blockIdx.x; it < items; it += gridDim.x) {
+        const int k = it / a.nfb, fblock = it - k * a.nfb;
+        const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n);
+        fftWindow(a, k, fBegin, fEnd, lds, zg);
+        __syncthreads();   // the item's epilogue read the workspace
+    }
 }
 
 // the persistent form over the work queue of one search-window size (every clip of a batch with this
@@ -243,7 +266,8 @@ __global__ void __launch_bounds__(kFftThreads) fftQueue(WorkQueue q, const FftAr
         if (it < 0) break;
         int c, k, fBegin, fEnd;
         q.locate(it, c, k, fBegin, fEnd);
-        fftWindow(clips[c], k, fBegin, fEnd, lds);
+        const FftArgs &a = clips[c];
+        fftWindow(a, k, fBegin, fEnd, lds, a.work ? a.work + size_t(blockIdx.x) * a.workStride : nullptr);
     }
 }
 
@@ -258,8 +282,9 @@ void configure(const void *kern)
 bool fftPlanFor(int L, FftPlan &plan)
 {
     plan.nf = 0;
+    if (L < 2 || L > kFftMaxL) return false;
     int m = L;
-    // radix 8 first (fewest passes), then 4, 2, 5, 3, then any odd factor up to kMaxRadix
+    // radix 8 first (fewest passes), then 4, 2, 5, 3, then the remaining primes (direct passes)
     static const int first[5] = {8, 4, 2, 5, 3};
     for (int r : first)
         while (m % r == 0) {
@@ -267,20 +292,42 @@ bool fftPlanFor(int L, FftPlan &plan)
             plan.fac[plan.nf++] = r;
             m /= r;
         }
-    for (int r = 7; m > 1 && r <= kMaxRadix; r += 2)
+    for (int r = 7; m > 1; r += 2)
         while (m % r == 0) {
             if (plan.nf == kMaxFactors) return false;
             plan.fac[plan.nf++] = r;
             m /= r;
         }
-    return m == 1 && fftShmem(L) <= 160 * 1024;
+    return true;
 }
 
-size_t fftShmem(int L)
+namespace {
+
+size_t fftTables(int L)
 {
-    const int np = (L + 1) / 2, rs = 2 * (L / 2 + 1);
-    return sizeof(float2) * size_t(np) * rs + sizeof(float2) * L + sizeof(double) * (kFftThreads / kWave)
-        + sizeof(int) * L + (sizeof(float) + 2 * sizeof(int)) * (kFftThreads / kWave);
+    return sizeof(float2) * L + sizeof(double) * (kFftThreads / kWave) + sizeof(int) * L
+        + (sizeof(float) + 2 * sizeof(int)) * (kFftThreads / kWave);
+}
+
+size_t fftSpectrum(int L)
+{
+    const size_t np = (L + 1) / 2, rs = 2 * (L / 2 + 1);
+    return sizeof(float2) * np * rs;
+}
+
+} // namespace
+
+bool fftInLds(int L) { return fftSpectrum(L) + fftTables(L) <= 160 * 1024; }
+
+size_t fftShmem(int L) { return fftTables(L) + (fftInLds(L) ? fftSpectrum(L) : 0); }
+
+size_t fftWorkPerBlock(int L) { return fftInLds(L) ? 0 : fftSpectrum(L); }
+
+int fftWorkBlocks(int L, int cus)
+{
+    // two blocks per CU, within 512 MiB of workspace
+    const size_t per = std::max<size_t>(fftSpectrum(L), 1);
+    return int(std::max<size_t>(1, std::min<size_t>(size_t(2) * cus, (size_t(512) << 20) / per)));
 }
 
 void launchFftReference(const FftArgs &a, hipStream_t stream)
@@ -312,7 +359,18 @@ void launchFftCorrelate(FftArgs a, hipStream_t stream)
     a.nfb = (a.fg.n + a.fpb - 1) / a.fpb;
     const int64_t blocks = int64_t(a.g.nwin) * a.nfb;
     if (blocks > 0x7fffffff) throw std::invalid_argument("framecor: too many windows x frames in one group");
-    hipLaunchKernelGGL(fftCorrelate, dim3(unsigned(blocks)), dim3(kFftThreads), fftShmem(a.g.L), stream, a);
+    if (a.work) {
+        static bool configuredGlobal = false;
+        if (!configuredGlobal) {
+            configure(reinterpret_cast<const void *>(fftCorrelateGlobal));
+            configuredGlobal = true;
+        }
+        const int grid = int(std::min<int64_t>(blocks, a.workBlocks));
+        hipLaunchKernelGGL(fftCorrelateGlobal, dim3(unsigned(grid)), dim3(kFftThreads), fftShmem(a.g.L), stream, a);
+    }
+    else {
+        hipLaunchKernelGGL(fftCorrelate, dim3(unsigned(blocks)), dim3(kFftThreads), fftShmem(a.g.L), stream, a);
+    }
     FC_HIP_CHECK(hipGetLastError());
 }
 

@@ -20,8 +20,6 @@
 
 namespace framecor {
 
-constexpr int kMaxRadix = 32;
-
 // floor(x / d) for 0 <= x < 2^20 and 1 <= d < 2^12 as a multiply-high by m = ceil(2^32 / d): the error
 // of x m / 2^32 against x / d is below x / 2^32 < 1 / d, which cannot carry it past the next integer
 __host__ __device__ inline unsigned magicOf(unsigned d) { return unsigned((0x100000000ull + d - 1) / d); }
@@ -185,39 +183,55 @@ __host__ __device__ inline void radixPass(float2 *Z, const float2 *tw, int L, in
     }
 }
 
-// a pass of any radix up to kMaxRadix, by direct sums with the twiddle table
-template <bool INV, class Addr>
-__host__ __device__ inline void genericPass(int R, float2 *Z, const float2 *tw, int L, int nlines, int Lj,
-                                            Addr addr, int tid, int nt)
+// a pass of any radix R by direct sums with the twiddle table (the radices without a register butterfly:
+// primes from 7 up, of any size). An item is one output element; the items are taken in rounds of whole
+// lines, at most IPT per thread, so that every input of a round is read (into registers) before any
+// output of it is written: the pass is in place. Needs IPT nt >= L (a line in one round).
+template <int IPT, bool INV, class Addr, class Sync>
+__host__ __device__ inline void directPass(int R, float2 *Z, const float2 *tw, int L, int nlines, int Lj, Addr addr,
+                                           int tid, int nt, Sync sync)
 {
-    const int Sj = Lj / R, per = L / R, tstep = L / Lj, rstep = L / R;
-    const int items = nlines * per;
-    const unsigned mPer = magicOf(per), mSj = magicOf(Sj);
-    for (int it = tid; it < items; it += nt) {
-        const int line = divMagic(it, mPer, per), w = it - line * per;
-        const int b = divMagic(w, mSj, Sj), n1 = w - b * Sj;
-        const int e0 = b * Lj + n1;
-        float2 v[kMaxRadix], o[kMaxRadix];
-        for (int m = 0; m < R; m++) v[m] = Z[addr(line, e0 + Sj * m)];
-        if (INV)
-            for (int c = 1; c < R; c++) v[c] = cmulc(v[c], tw[n1 * c * tstep]);
-        for (int c = 0; c < R; c++) {
-            float2 acc = make_float2(0.0f, 0.0f);
-            for (int m = 0; m < R; m++) {
-                const float2 t = tw[((m * c) % R) * rstep];
-                acc = cadd(acc, INV ? cmulc(v[m], t) : cmul(v[m], t));
+    const int Sj = Lj / R, tstep = L / Lj, rstep = L / R;
+    const int linesPerRound = (IPT * nt) / L > 0 ? (IPT * nt) / L : 1;
+    const unsigned mL = magicOf(L), mLj = magicOf(Lj), mSj = magicOf(Sj);
+    for (int l0 = 0; l0 < nlines; l0 += linesPerRound) {
+        const int l1 = l0 + linesPerRound < nlines ? l0 + linesPerRound : nlines;
+        const int items = (l1 - l0) * L;
+        float2 o[IPT];
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+            const int it = tid + i * nt;
+            if (it < items) {
+                const int ll = divMagic(it, mL, L), line = l0 + ll, e = it - ll * L;
+                const int b = divMagic(e, mLj, Lj), r = e - b * Lj;
+                const int c = divMagic(r, mSj, Sj), n1 = r - c * Sj;
+                const int e0 = b * Lj + n1;
+                float2 acc = make_float2(0.0f, 0.0f);
+                for (int m = 0; m < R; m++) {
+                    float2 v = Z[addr(line, e0 + Sj * m)];
+                    if (INV && m > 0) v = cmulc(v, tw[n1 * m * tstep]);
+                    const float2 w = tw[((m * c) % R) * rstep];
+                    acc = cadd(acc, INV ? cmulc(v, w) : cmul(v, w));
+                }
+                o[i] = (!INV && c > 0) ? cmul(acc, tw[n1 * c * tstep]) : acc;
             }
-            o[c] = acc;
         }
-        if (!INV)
-            for (int c = 1; c < R; c++) o[c] = cmul(o[c], tw[n1 * c * tstep]);
-        for (int m = 0; m < R; m++) Z[addr(line, e0 + Sj * m)] = o[m];
+        sync();
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+            const int it = tid + i * nt;
+            if (it < items) {
+                const int ll = divMagic(it, mL, L);
+                Z[addr(l0 + ll, it - ll * L)] = o[i];
+            }
+        }
+        sync();
     }
 }
 
-template <bool INV, class Addr>
+template <bool INV, class Addr, class Sync>
 __host__ __device__ inline void anyPass(int R, float2 *Z, const float2 *tw, int L, int nlines, int Lj, Addr addr,
-                                        int tid, int nt)
+                                        int tid, int nt, Sync sync)
 {
     switch (R) {
     case 2: radixPass<2, INV>(Z, tw, L, nlines, Lj, addr, tid, nt); break;
@@ -225,7 +239,7 @@ __host__ __device__ inline void anyPass(int R, float2 *Z, const float2 *tw, int 
     case 4: radixPass<4, INV>(Z, tw, L, nlines, Lj, addr, tid, nt); break;
     case 5: radixPass<5, INV>(Z, tw, L, nlines, Lj, addr, tid, nt); break;
     case 8: radixPass<8, INV>(Z, tw, L, nlines, Lj, addr, tid, nt); break;
-    default: genericPass<INV>(R, Z, tw, L, nlines, Lj, addr, tid, nt); break;
+    default: directPass<4, INV>(R, Z, tw, L, nlines, Lj, addr, tid, nt, sync); break;
     }
 }
 
@@ -238,7 +252,7 @@ __host__ __device__ inline void transformLines(float2 *Z, const float2 *tw, int 
     if (!INV) {
         int Lj = L;
         for (int j = 0; j < nf; j++) {
-            anyPass<false>(fac[j], Z, tw, L, nlines, Lj, addr, tid, nt);
+            anyPass<false>(fac[j], Z, tw, L, nlines, Lj, addr, tid, nt, sync);
             Lj /= fac[j];
             sync();
         }
@@ -247,7 +261,7 @@ __host__ __device__ inline void transformLines(float2 *Z, const float2 *tw, int 
         for (int j = nf - 1; j >= 0; j--) {
             int Lj = L;
             for (int i = 0; i < j; i++) Lj /= fac[i];
-            anyPass<true>(fac[j], Z, tw, L, nlines, Lj, addr, tid, nt);
+            anyPass<true>(fac[j], Z, tw, L, nlines, Lj, addr, tid, nt, sync);
             sync();
         }
     }

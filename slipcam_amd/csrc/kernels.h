@@ -134,11 +134,26 @@ struct FftArgs {
     int H;                   // L / 2 + 1, the half-spectrum width
     int rs;                  // complex slots per row pair: 2 H
     int fpb, nfb;            // frames per block, frame blocks
+    // search windows whose half spectrum does not fit in LDS: each block's spectrum in a workspace in
+    // global memory (np x rs complex per block, block b at work + b * workStride), and a persistent grid
+    // of workBlocks blocks; null when the spectrum is held in LDS
+    float2 *work;
+    size_t workStride;
+    int workBlocks;
 };
 
-// whether the FFT form supports L (factors <= 32 and the half spectrum in LDS), and its plan
+// the largest search window of the FFT form (a line of a transform pass fits in one round of a block)
+constexpr int kFftMaxL = 2048;
+// the FFT form's plan of L (radices 8, 4, 2, 5, 3 and then any prime; L <= kFftMaxL), false if none
 bool fftPlanFor(int L, FftPlan &plan);
+// whether the half spectrum of L is held in LDS (else in a global workspace)
+bool fftInLds(int L);
+// the LDS of a block: the half spectrum when it fits, plus the twiddle and position tables
 size_t fftShmem(int L);
+// the workspace of one block when the spectrum is not in LDS (bytes; 0 if it is), and the persistent grid
+// on a device of cus compute units
+size_t fftWorkPerBlock(int L);
+int fftWorkBlocks(int L, int cus);
 // the spectra of the reference windows (args: g, wd, twiddle, templ, spectraOut, plan, np, H, rs)
 void launchFftReference(const FftArgs &a, hipStream_t stream);
 // correlate a group of frames

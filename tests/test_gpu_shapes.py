@@ -4,9 +4,10 @@ Parity of both GPU forms over the range of geometries they support, against the 
   int8   every tile shape family (template column blocks 2, 3, 5, 9; 1 to 9 bands; edge waves or
          padded bands; the largest window 129 and search 71) and every kernel variant of each, against
          the FP64 direct-NCC oracle, with gross offsets that push some search windows out of the frame
-  fft    search windows whose size has factors the radix-2/4/8 passes do not cover (3, 5, 7, 11, a
-         prime 29) and the largest one held in LDS (192), against the restatement of the reference
-         algorithm; a size with a prime factor above 31 is refused
+  fft    search windows whose size has factors the radix-2/4/8 passes do not cover (3, 5, 7, 11, the
+         primes 29, 37, 67 and 197: direct passes), the largest ones held in LDS (192, 197) and larger ones
+         whose spectra go to the global workspace (201, 256), against the restatement of the reference
+         algorithm; a size above 2048 is refused
   both   every window invalid; a flat reference
 """
 
@@ -15,7 +16,7 @@ import pytest
 
 from oracle import OracleCorrelator
 from slipcam_amd import framecor, synthetic
-from test_gpu_parity import assert_fft_parity, assert_int8_parity
+from parity import assert_fft_parity, assert_int8_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -54,7 +55,11 @@ def test_int8_shapes_vs_fp64_oracle(gpu, W, R):
         assert_int8_parity(got, ref)
 
 
-FFT_SHAPES = [(15, 7), (21, 7), (24, 10), (33, 22), (16, 40), (64, 64)]   # L = 29, 35, 44, 77, 96, 192
+# L = 29, 35, 44, 77, 96, 192 (the largest power-of-two-and-3 size in LDS), 37 and 197 (primes: direct
+# passes; 197 the largest odd size in LDS), 256 (window 128 and 160: spectra in the global workspace),
+# 201 = 3 x 67 (global workspace and a direct pass of 67)
+FFT_SHAPES = [(15, 7), (21, 7), (24, 10), (33, 22), (16, 40), (64, 64), (25, 6), (97, 50), (128, 64), (160, 48),
+              (101, 50)]
 
 
 @pytest.mark.parametrize('W,R', FFT_SHAPES)
@@ -71,10 +76,9 @@ def test_fft_shapes_vs_reference_algorithm(gpu, W, R):
     assert assert_fft_parity(out, ref, R, surf=surf, gross=d['gross']) <= 1
 
 
-def test_fft_refuses_a_large_prime_factor(gpu):
-    down, across = framecor.window_grid(200, 200, 25, 6, 20)      # L = 37
-    with pytest.raises(ValueError, match='prime factors'):
-        framecor.Correlator(200, 200, down, across, window=25, search=6, method='fft')
+def test_fft_refuses_a_search_window_beyond_2048(gpu):
+    with pytest.raises(ValueError, match='at most 2048'):
+        framecor.Correlator(2400, 2400, [1100], [1100], window=49, search=1000, method='fft')
 
 
 @pytest.mark.parametrize('method', ['fft', 'int8'])

@@ -94,10 +94,12 @@ def test_arguments_without_gpu():
     # what the methods support
     with pytest.raises(ValueError, match='int8'):
         framecor.Correlator(1000, 1000, [0], [0], window=160, search=8, method='int8')
-    with pytest.raises(ValueError, match='fft'):
-        framecor.Correlator(1000, 1000, [0], [0], window=3, search=22, method='fft')     # L = 47, prime
-    with pytest.raises(ValueError, match='fft'):
-        framecor.Correlator(1000, 1000, [0], [0], window=128, search=64, method='fft')   # L = 256 > 192
+    with pytest.raises(ValueError, match='fft method needs a search window .* of at most 2048'):
+        framecor.Correlator(3000, 3000, [0], [0], window=181, search=1000, method='fft')   # L = 2181
+    # any L up to 2048 passes the checks (L = 47, a prime; L = 256, beyond LDS)
+    for W, R in ((3, 22), (128, 64), (160, 48)):
+        assert framecor._lib.framecor_check_params(1000, 1000, np.zeros(1, np.int32).ctypes.data, 1,
+                                                   np.zeros(1, np.int32).ctypes.data, 1, W, R, 0, 1, 21) == 0
 
 
 @pytest.mark.skipif(framecor.has_gpu(), reason='a GPU is present')
@@ -132,7 +134,7 @@ def test_batch_arguments_without_gpu():
     with pytest.raises(ValueError, match=r'outside the frame \(across 400\) \(clip 2\)'):
         framecor.Batch([good, good, dict(good, across=[400])], method='int8')
     with pytest.raises(ValueError, match=r'fft method .* \(clip 0\)'):
-        framecor.Batch([dict(good, height=1000, width=1000, window=128, search=64)], method='fft')
+        framecor.Batch([dict(good, height=3000, width=3000, window=128, search=1000)], method='fft')
     with pytest.raises(ValueError, match='frames must not be negative'):
         framecor.Batch([dict(good, frames=-1)], method='int8')
     with pytest.raises(ValueError, match='method'):
