@@ -118,7 +118,7 @@ struct framecor_batch {
             sh.shmem = k.shmem;
         }
         else {
-            const FftKernel k = fftQueueKernel(sh.L);
+            const FftKernel k = fftFastFor(sh.L) ? fftFastQueueKernel(sh.L) : fftQueueKernel(sh.L);
             sh.func = k.func;
             sh.threads = k.threads;
             sh.shmem = k.shmem;

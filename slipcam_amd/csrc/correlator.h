@@ -392,6 +392,11 @@ struct framecor_correlator {
             a.spectra = dSpectra.as<float2>();
             a.twiddle = dTwiddle.as<float2>();
             setWork(a);
+            if (fftFastFor(g.L)) {
+                launchFftFastCorrelate(a, s);
+                if (timed) mark(s);
+                return;
+            }
             a.plan = plan;
             a.np = (g.L + 1) / 2;
             a.H = g.L / 2 + 1;
@@ -715,7 +720,10 @@ struct framecor_correlator {
             a.np = (g.L + 1) / 2;
             a.H = g.L / 2 + 1;
             a.rs = 2 * a.H;
-            launchFftReference(a, stream);
+            if (fftFastFor(g.L))
+                launchFftFastReference(a, stream);
+            else
+                launchFftReference(a, stream);
             FC_HIP_CHECK(hipStreamSynchronize(stream));
         }
         else {

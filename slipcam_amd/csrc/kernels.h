@@ -159,6 +159,13 @@ void launchFftReference(const FftArgs &a, hipStream_t stream);
 // correlate a group of frames
 void launchFftCorrelate(FftArgs a, hipStream_t stream);
 
+// the register-resident form of the FFT correlation for L = 64, 96, 128, 160, 192 (fft_fast.hip); its
+// reference spectra have their own layout (the same size, nwin x (L / 2 + 1) x L complex)
+bool fftFastFor(int L);
+size_t fftFastShmem(int L);
+void launchFftFastReference(const FftArgs &a, hipStream_t stream);
+void launchFftFastCorrelate(const FftArgs &a, hipStream_t stream);
+
 // the persistent work-queue kernel of the FFT form (search-window size L: its LDS)
 struct FftKernel {
     const void *func;
@@ -166,6 +173,7 @@ struct FftKernel {
     size_t shmem;
 };
 FftKernel fftQueueKernel(int L);
+FftKernel fftFastQueueKernel(int L);
 // clips: device array of the clips' arguments (np, H, rs, plan, spectra, twiddle filled)
 void launchFftQueue(const FftKernel &k, const WorkQueue &q, const FftArgs *clips, int blocks, hipStream_t stream);
 
