@@ -101,9 +101,10 @@ int framecor_correlate_device(framecor_correlator *c, const uint8_t *frames, int
  * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc). The box-sum maps of each group of frames go to a static 64 MiB device array per GPU,
  * held until the process ends, instead of buffers the correlator allocates; frames are correlated in
- * groups whose maps fit in it (12 bytes per window position and frame for the int8 method, 8 for the
- * FFT: 64 MiB holds 6 frames at 720 x 1280, window 64). FRAMECOR_EINVAL if one frame's maps exceed
- * 64 MiB. Every call on a GPU shares that GPU's array: calls are ordered on the GPU (each waits for the
+ * groups whose maps fit in it (12 bytes per window position and frame for the int8 method, 4 for the
+ * FFT: 64 MiB holds 6 or 21 frames at 720 x 1280, window 64). FRAMECOR_EINVAL if one frame's maps
+ * exceed 64 MiB, and for the FFT method when its spectra do not fit in LDS (window + 2 search above
+ * 197: the correlator's own workspace is needed). Every call on a GPU shares that GPU's array: calls are ordered on the GPU (each waits for the
  * previous one's work on it), except under stream capture, where captured work that overlaps other
  * calls on the same GPU races. The correlator must have been created and given its reference (those
  * calls allocate). Arguments and results as framecor_correlate_device. */

@@ -153,9 +153,9 @@ struct framecor_batch {
         FrameGroup fg{};
         fg.frames = fr + size_t(first) * k.g.pixels;
         fg.gross = gross ? gross + 2 * q0 : nullptr;
-        fg.box = isInt8() ? nullptr : k.dBox.as<int32_t>();
-        fg.box2 = k.dBox2.as<int32_t>();
+        fg.box2 = isInt8() ? k.dBox2.as<int32_t>() : nullptr;
         fg.screen = isInt8() ? k.dScreen.as<int2>() : nullptr;
+        fg.sigma = isInt8() ? nullptr : k.dSigma.as<float>();
         fg.n = count;
         fg.offset = offset + 2 * q0;
         fg.peak = peak + q0;
@@ -293,9 +293,9 @@ struct framecor_batch {
                 int first, count;
                 stageFrames(int(c), st, first, count);
                 if (count <= 0) continue;
-                launchBoxSums(fr[c] + size_t(first) * k.g.pixels, count, k.g.h, k.g.w, k.g.W,
-                              isInt8() ? nullptr : k.dBox.as<int32_t>(), k.dBox2.as<int32_t>(),
-                              isInt8() ? k.dScreen.as<int2>() : nullptr, s);
+                launchBoxSums(fr[c] + size_t(first) * k.g.pixels, count, k.g.h, k.g.w, k.g.W, nullptr,
+                              isInt8() ? k.dBox2.as<int32_t>() : nullptr, isInt8() ? k.dScreen.as<int2>() : nullptr,
+                              isInt8() ? nullptr : k.dSigma.as<float>(), s);
             }
             for (const Shape &sh : shapes)
                 for (WorkQueue q : queues(sh, st)) {
@@ -322,8 +322,9 @@ struct framecor_batch {
                 int first, count;
                 stageFrames(int(c), 0, first, count);
                 if (count > 0)
-                    launchBoxSums(fr[c], count, k.g.h, k.g.w, k.g.W, isInt8() ? nullptr : k.dBox.as<int32_t>(),
-                                  k.dBox2.as<int32_t>(), isInt8() ? k.dScreen.as<int2>() : nullptr, stream);
+                    launchBoxSums(fr[c], count, k.g.h, k.g.w, k.g.W, nullptr, isInt8() ? k.dBox2.as<int32_t>() : nullptr,
+                                  isInt8() ? k.dScreen.as<int2>() : nullptr, isInt8() ? nullptr : k.dSigma.as<float>(),
+                                  stream);
             }
             for (Shape &sh : shapes) {
                 if (sh.tuned) continue;

@@ -217,7 +217,7 @@ int framecor_box_sums_device(const uint8_t *frames, int n, int height, int width
             invalid("the window must be 1 .. 181 pixels and fit in the frame");
         if (n > 65535) invalid("at most 65535 frames per call");
         if (n > 0 && (!frames || !box || !box2)) invalid("null argument");
-        launchBoxSums(frames, n, height, width, window, box, box2, nullptr, static_cast<hipStream_t>(stream));
+        launchBoxSums(frames, n, height, width, window, box, box2, nullptr, nullptr, static_cast<hipStream_t>(stream));
     });
 }
 

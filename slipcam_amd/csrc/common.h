@@ -39,8 +39,9 @@ struct WindowData {
 struct FrameGroup {
     const uint8_t *frames;      // n x h x w
     const int *gross;           // n x nwin x 2, or null
-    const int32_t *box, *box2;  // n x bh x bw: W x W box sums of the pixels (FFT form) and of their squares
+    const int32_t *box2;        // n x bh x bw: W x W box sums of the squares of the pixels (int8 form)
     const int2 *screen;         // n x bh x bw: {box, 1 / sqrt(W^2 box2 - box^2) float bits} (int8 form)
+    const float *sigma;         // n x bh x bw: sqrtf(float(W^2 box2 - box^2) / float(W^2)), 0 if flat (FFT form)
     int n;
     float *offset, *peak, *snr, *cov;
 };

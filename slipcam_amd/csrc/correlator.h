@@ -184,7 +184,7 @@ struct framecor_correlator {
     DeviceBuffer dWork;
     int workBlocks = 0;
     // per group of frames
-    DeviceBuffer dBox, dBox2, dScreen;
+    DeviceBuffer dBox2, dScreen, dSigma;
     int groupCap = 0;
     // host-call staging
     DeviceBuffer dFrames, dGross, dOffset, dPeak, dSnr, dCov;
@@ -322,8 +322,8 @@ struct framecor_correlator {
 
     size_t bytesPerFrame() const
     {
-        // int8: the screen map {box, 1 / sigma} and box2; fft: box and box2
-        return size_t(g.boxSize) * (method == FRAMECOR_METHOD_INT8 ? 12 : 8);
+        // int8: the screen map {box, 1 / sigma} and box2; fft: sigma
+        return size_t(g.boxSize) * (method == FRAMECOR_METHOD_INT8 ? 12 : 4);
     }
 
     void ensureGroup(int n)
@@ -331,8 +331,8 @@ struct framecor_correlator {
         const int want = std::min(n, maxGroup);
         if (want <= groupCap) return;
         const bool isInt8 = method == FRAMECOR_METHOD_INT8;
-        if (!isInt8) dBox.ensure(size_t(want) * g.boxSize * sizeof(int32_t));
-        dBox2.ensure(size_t(want) * g.boxSize * sizeof(int32_t));
+        if (isInt8) dBox2.ensure(size_t(want) * g.boxSize * sizeof(int32_t));
+        else dSigma.ensure(size_t(want) * g.boxSize * sizeof(float));
         if (isInt8) dScreen.ensure(size_t(want) * g.boxSize * sizeof(int2));
         groupCap = want;
     }
@@ -352,25 +352,27 @@ struct framecor_correlator {
     void runGroup(const uint8_t *frames, int n, const int32_t *gross, float *offset, float *peak, float *snr,
                   float *cov, hipStream_t s)
     {
-        runGroupWith(frames, n, gross, offset, peak, snr, cov, s, dBox.as<int32_t>(), dBox2.as<int32_t>(),
-                     dScreen.as<int2>(), true);
+        runGroupWith(frames, n, gross, offset, peak, snr, cov, s, dBox2.as<int32_t>(), dScreen.as<int2>(),
+                     dSigma.as<float>(), true);
     }
 
-    // the same with given box-map buffers (int8: screen and box2; FFT: box and box2, n x boxSize each);
+    // the same with given box-map buffers (int8: box2 and screen; FFT: sigma, n x boxSize each);
     // timed: the per-stage events of set_timing
     void runGroupWith(const uint8_t *frames, int n, const int32_t *gross, float *offset, float *peak, float *snr,
-                      float *cov, hipStream_t s, int32_t *box, int32_t *box2, int2 *screen, bool timed)
+                      float *cov, hipStream_t s, int32_t *box2, int2 *screen, float *sigma, bool timed)
     {
         const bool isInt8 = method == FRAMECOR_METHOD_INT8;
+        if (!isInt8) box2 = nullptr, screen = nullptr;
+        else sigma = nullptr;
         if (timed) mark(s);
-        launchBoxSums(frames, n, g.h, g.w, g.W, isInt8 ? nullptr : box, box2, isInt8 ? screen : nullptr, s);
+        launchBoxSums(frames, n, g.h, g.w, g.W, nullptr, box2, screen, sigma, s);
         if (timed) mark(s);
         FrameGroup fg;
         fg.frames = frames;
         fg.gross = gross;
-        fg.box = isInt8 ? nullptr : box;
         fg.box2 = box2;
-        fg.screen = isInt8 ? screen : nullptr;
+        fg.screen = screen;
+        fg.sigma = sigma;
         fg.n = n;
         fg.offset = offset;
         fg.peak = peak;
@@ -445,22 +447,22 @@ struct framecor_correlator {
         const int G = int(std::min<size_t>(size_t(maxGroup), bytes / perFrame));
         const size_t plane = size_t(G) * g.boxSize;
         unsigned char *base = static_cast<unsigned char *>(scratch);
-        int32_t *box = nullptr, *box2 = nullptr;
+        int32_t *box2 = nullptr;
         int2 *screen = nullptr;
+        float *sigma = nullptr;
         if (method == FRAMECOR_METHOD_INT8) {
             screen = reinterpret_cast<int2 *>(base);
             box2 = reinterpret_cast<int32_t *>(base + plane * sizeof(int2));
         }
         else {
-            box = reinterpret_cast<int32_t *>(base);
-            box2 = reinterpret_cast<int32_t *>(base + plane * sizeof(int32_t));
+            sigma = reinterpret_cast<float *>(base);
         }
         after(s);
         for (int f0 = 0; f0 < n; f0 += G) {
             const int nf = std::min(G, n - f0);
             const size_t q0 = size_t(f0) * g.nwin;
             runGroupWith(frames + size_t(f0) * g.pixels, nf, gross ? gross + 2 * q0 : nullptr, offset + 2 * q0,
-                         peak + q0, snr + q0, cov + 3 * q0, s, box, box2, screen, false);
+                         peak + q0, snr + q0, cov + 3 * q0, s, box2, screen, sigma, false);
         }
         done(s);
     }
@@ -534,7 +536,7 @@ struct framecor_correlator {
     // (dSurfBox, dSurfBox2: ensured by the caller, before it orders its stream)
     void surfaceBoxes(const uint8_t *frame, hipStream_t s)
     {
-        launchBoxSums(frame, 1, g.h, g.w, g.W, dSurfBox.as<int32_t>(), dSurfBox2.as<int32_t>(), nullptr, s);
+        launchBoxSums(frame, 1, g.h, g.w, g.W, dSurfBox.as<int32_t>(), dSurfBox2.as<int32_t>(), nullptr, nullptr, s);
     }
 
     void launchSurfaces(const uint8_t *frame, const int32_t *gross, int k0, int count, double *surf, hipStream_t s)

@@ -164,24 +164,22 @@ __device__ __attribute__((always_inline)) inline void fftWindow(const FftArgs &a
         combinePairs<kIpt>(s.Z, s.pos, L, a.H, a.rs, npOut, tid, kFftThreads, sync);
         transformLines<true>(s.Z, s.tw, L, a.plan.fac, a.plan.nf, npOut, RowAddr{a.rs}, tid, kFftThreads, sync);
 
-        // normalize in place and find the first maximum
+        // normalize in place and find the first maximum: c / L^2 over norm * sqrtf(float(var) / float(n)),
+        // the second factor from the frame's sigma map (0 where var <= 0)
         const float scale = 1.0f / (float(L) * float(L));
         const int n = W * W;
-        const int32_t *box = a.fg.box + f * g.boxSize + int64_t(sy) * g.bw + sx;
-        const int32_t *box2 = a.fg.box2 + f * g.boxSize + int64_t(sy) * g.bw + sx;
+        const float *sig = a.fg.sigma + f * g.boxSize + int64_t(sy) * g.bw + sx;
         float best = -2.0f;
         int ib = 0x7fffffff;
         for (int idx = tid; idx < M * M; idx += kFftThreads) {
             const int u = idx / M, v = idx - u * M;
             float &c = zf[2 * ((u >> 1) * a.rs + v) + (u & 1)];
-            const int64_t o = int64_t(u) * g.bw + v;
-            const int64_t bs = box[o], bs2 = box2[o];
-            const int64_t var = int64_t(n) * bs2 - bs * bs;
+            const float sg = sig[int64_t(u) * g.bw + v];
             float val = 0.0f;
             {
 #pragma clang fp contract(off)
                 const float cc = c * scale;
-                if (var > 0 && normk > 0) val = cc / (normk * sqrtf(float(var) / float(n)));
+                if (sg > 0 && normk > 0) val = cc / (normk * sg);
             }
             c = val;
             bestOf(best, ib, val, idx);

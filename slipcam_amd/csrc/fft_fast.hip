@@ -4,7 +4,7 @@
 // The correlation of one search window is the reference's (cxx/cpu.cpp:117-167): the L x L chip minus
 // its mean, its 2-D real FFT, the product with the conjugate spectrum of the zero-padded reference
 // window, the inverse transform, normalization (cxx/surface.h:30-35), first maximum and writePeak.
-// One block of NW waves takes one window and a run of frames; per frame, in LDS:
+// One block of NW waves (8 to 16) takes one window and a run of frames; per frame, in LDS:
 //
 //   S      L rows of L / 2 complex (row stride P, plus one slot per 16 rows: bank-conflict free column
 //          reads), the half spectrum with the DC and Nyquist columns packed into slot 0 (both are real
@@ -269,6 +269,91 @@ __device__ inline int partnerRow(int k2, int k1)
     return 16 * k2p + k1p;
 }
 
+// inverse step 1 of column k: task j = l, the output rows j + 16 m < M
+template <class F>
+__device__ inline void colInverseStep1(float2 *S, int k, int l, int M)
+{
+    constexpr int N2 = F::N2;
+    const int j = l;
+    float2 *col = S + k + F::idx(j, 0);
+    float2 v[N2];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) v[k2] = col[k2 * (16 * F::P + 1)];
+    dftReg<N2, true>(v);
+#pragma unroll
+    for (int m = 0; m < N2; m++)
+        if (j + 16 * m < M) col[m * (16 * F::P + 1)] = v[m];
+}
+
+// inverse step 2 of a column (task k2, the product in u): element (j, k2) at row 16 k2 + j
+template <class F>
+__device__ inline void colInverseStep2(float2 *S, const float2 *tw, int k, int k2, float2 (&u)[16])
+{
+    dftReg<16, true>(u);
+#pragma unroll
+    for (int j = 1; j < 16; j++) u[j] = cmulc(u[j], tw[j * k2]);
+    float2 *cq = S + k + k2 * (16 * F::P + 1);
+#pragma unroll
+    for (int j = 0; j < 16; j++) cq[j * F::P] = u[j];
+}
+
+// column k >= 1 of the correlation: forward transform, times the conjugate reference spectrum (16 values
+// per task, read from global memory after the forward transform), inverse transform, the M output rows
+template <class F>
+__device__ inline void columnTask(float2 *S, const float2 *tw, const float2 *spec, int kc, int l, int M)
+{
+    constexpr int N2 = F::N2;
+    colStep1<F>(S, tw, kc, l);
+    waveFence();
+    const int k2 = l;
+    if (k2 < N2) {
+        float2 u[16];
+        colStep2<F>(S, kc, k2, u);
+        const float4 *s4 = reinterpret_cast<const float4 *>(spec + (size_t(kc) * N2 + k2) * 16);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float4 sp = s4[i];
+            u[2 * i] = cmul(u[2 * i], make_float2(sp.x, sp.y));
+            u[2 * i + 1] = cmul(u[2 * i + 1], make_float2(sp.z, sp.w));
+        }
+        colInverseStep2<F>(S, tw, kc, k2, u);
+    }
+    waveFence();
+    colInverseStep1<F>(S, kc, l, M);
+}
+
+// column 0, the DC and Nyquist columns packed as D + i N: C'(f) = C(f) P(f) + conj C(L - f) Q(f) with
+// P, Q from the reference spectra, which is D(f) conj TD(f) + i N(f) conj TN(f); C(L - f) is read back
+// from the lanes that hold it
+template <class F>
+__device__ inline void packedColumnTask(float2 *S, const float2 *tw, const float2 *spec, int l, int M)
+{
+    constexpr int N2 = F::N2, H = F::H, P = F::P;
+    colStep1<F>(S, tw, 0, l);
+    waveFence();
+    const int k2 = l;
+    float2 u[16];
+    if (k2 < N2) {
+        colStep2<F>(S, 0, k2, u);
+        float2 *cq = S + k2 * (16 * P + 1);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; k1++) cq[k1 * P] = u[k1];
+    }
+    waveFence();
+    if (k2 < N2) {
+        const float2 *pv = spec + size_t(k2) * 16, *qv = spec + (size_t(H) * N2 + k2) * 16;
+#pragma unroll 4
+        for (int k1 = 0; k1 < 16; k1++) {
+            const float2 zc = S[F::idx(partnerRow<F>(k2, k1), 0)];
+            u[k1] = cadd(cmul(u[k1], pv[k1]), cmul(make_float2(zc.x, -zc.y), qv[k1]));
+        }
+    }
+    waveFence();
+    if (k2 < N2) colInverseStep2<F>(S, tw, 0, k2, u);
+    waveFence();
+    colInverseStep1<F>(S, 0, l, M);
+}
+
 // ---- the correlation of window k with frames fBegin .. fEnd - 1 (the whole block)
 
 template <class F>
@@ -326,11 +411,15 @@ __device__ __attribute__((always_inline)) inline void fastWindow(const FftArgs &
         // cxx/cpu.cpp:130-137: mean = float(box sum over L x L) / float(L^2), buf = px - mean
         const float mean = float(boxL) / (float(L) * float(L));
 
-        // ---- rows
+        // ---- rows (each stage takes the lane index anew: its lane-dependent addresses are made there, not
+        // kept in registers across the frame loop)
+        int lr = l;
+        asm volatile("" : "+v"(lr));
+        lr &= 15;
         for (int b = wave; 4 * b < H; b += NW) {
             const int p = 4 * b + grp;
             if (p < H)
-                rowForward<F>(S, tw, p, l, [mean](const float2 *pair, int x) {
+                rowForward<F>(S, tw, p, lr, [mean](const float2 *pair, int x) {
                     const uint8_t *ra = reinterpret_cast<const uint8_t *>(pair);
                     const uint8_t *rb = reinterpret_cast<const uint8_t *>(pair + P);
                     return make_float2(float(ra[x]) - mean, float(rb[x]) - mean);
@@ -339,84 +428,39 @@ __device__ __attribute__((always_inline)) inline void fastWindow(const FftArgs &
         }
         __syncthreads();
 
-        // ---- columns: forward, times the conjugate reference spectrum, inverse; the M output rows
-        for (int b = wave; 4 * b < H; b += NW) {
-            const int kc = 4 * b + grp;
-            if (kc < H) {
-                const int k2 = l;
-                colStep1<F>(S, tw, kc, l);
-                waveFence();
-                if (k2 < N2) {
-                    // this task's 16 spectrum values (issued before its LDS reads and transform)
-                    const float4 *s4 = reinterpret_cast<const float4 *>(spec + (size_t(kc) * N2 + k2) * 16);
-                    float4 sp[8];
-#pragma unroll
-                    for (int i = 0; i < 8; i++) sp[i] = s4[i];
-                    float2 u[16];
-                    colStep2<F>(S, kc, k2, u);
-                    float2 *cq = S + kc + k2 * (16 * P + 1);   // rows 16 k2 + j at j P
-                    if (kc != 0) {
-#pragma unroll
-                        for (int i = 0; i < 8; i++) {
-                            u[2 * i] = cmul(u[2 * i], make_float2(sp[i].x, sp[i].y));
-                            u[2 * i + 1] = cmul(u[2 * i + 1], make_float2(sp[i].z, sp[i].w));
-                        }
-                    }
-                    else {
-                        // the packed DC / Nyquist column: C'(f) = C(f) P(f) + conj C(L - f) Q(f), C(L - f) read
-                        // back from the lanes that hold it
-#pragma unroll
-                        for (int k1 = 0; k1 < 16; k1++) cq[k1 * P] = u[k1];
-                        waveFence();
-                        const float2 *qv = spec + (size_t(H) * N2 + k2) * 16;
-#pragma unroll
-                        for (int k1 = 0; k1 < 16; k1++) {
-                            const float2 zc = S[F::idx(partnerRow<F>(k2, k1), 0)];
-                            const float2 pv = (k1 & 1) ? make_float2(sp[k1 >> 1].z, sp[k1 >> 1].w)
-                                                       : make_float2(sp[k1 >> 1].x, sp[k1 >> 1].y);
-                            u[k1] = cadd(cmul(u[k1], pv), cmul(make_float2(zc.x, -zc.y), qv[k1]));
-                        }
-                        waveFence();
-                    }
-                    dftReg<16, true>(u);
-#pragma unroll
-                    for (int j = 1; j < 16; j++) u[j] = cmulc(u[j], tw[j * k2]);
-#pragma unroll
-                    for (int j = 0; j < 16; j++) cq[j * P] = u[j];
-                }
-                waveFence();
-                {   // inverse step 1: task j, the output rows j + 16 m < M
-                    const int j = l;
-                    float2 *col = S + kc + F::idx(j, 0);
-                    float2 v[N2];
-#pragma unroll
-                    for (int k2b = 0; k2b < N2; k2b++) v[k2b] = col[k2b * (16 * P + 1)];
-                    dftReg<N2, true>(v);
-#pragma unroll
-                    for (int m = 0; m < N2; m++)
-                        if (j + 16 * m < M) col[m * (16 * P + 1)] = v[m];
-                }
-            }
+        // ---- columns: forward, times the conjugate reference spectrum, inverse; the M output rows.
+        // Columns 1 .. H - 1 four to a wave at a time; the packed column 0 by the first lane group of the
+        // last wave (which has the fewest of the others) in a pass of its own
+        int lc = l;
+        asm volatile("" : "+v"(lc));
+        lc &= 15;
+        for (int b = wave; 4 * b < H - 1; b += NW) {
+            const int kc = 1 + 4 * b + grp;
+            if (kc < H) columnTask<F>(S, tw, spec, kc, lc, M);
             waveFence();
         }
+        if (wave == NW - 1 && grp == 0) packedColumnTask<F>(S, tw, spec, lc, M);
         __syncthreads();
 
         // ---- output row pairs: inverse real transforms, normalization, first maximum
         const float scale = 1.0f / (float(L) * float(L));
         const int n = W * W;
-        // the box sums of the surface points (u, x) at u bw + x (32-bit offsets from a uniform base)
-        const int32_t *box = a.fg.box + int64_t(f) * g.boxSize + int64_t(sy) * g.bw + sx;
-        const int32_t *box2 = a.fg.box2 + int64_t(f) * g.boxSize + int64_t(sy) * g.bw + sx;
+        // the sigma map sqrtf(float(var) / float(n)) of the surface points (u, x) at u bw + x (32-bit offsets
+        // from a uniform base; 0 where var <= 0)
+        const float *sig = a.fg.sigma + int64_t(f) * g.boxSize + int64_t(sy) * g.bw + sx;
         const int bw = g.bw;
         float best = -2.0f;
         int ib = 0x7fffffff;
         const int nq = (M + 1) / 2;
+        int li = l;
+        asm volatile("" : "+v"(li));
+        li &= 15;
         for (int b = wave; 4 * b < nq; b += NW) {
             const int qp = 4 * b + grp;
             if (qp < nq) {
                 float2 *pr = S + F::idx(2 * qp, 0);
                 const bool hasB = 2 * qp + 1 < M;
-                const int k2 = l;
+                const int k2 = li;
                 if (k2 < N2) {   // inverse step 2 with the combine: z(f) = R_a(f) + i R_b(f), Hermitian for f > L / 2
                     float2 u[16];
 #pragma unroll
@@ -435,6 +479,8 @@ __device__ __attribute__((always_inline)) inline void fastWindow(const FftArgs &
                             z = make_float2(sa.x + sb.y, sb.x - sa.y);
                         }
                         u[k1] = z;
+                        // the loads of eight elements at a time (fewer live registers)
+                        if (k1 == 7) __builtin_amdgcn_sched_barrier(0);
                     }
                     dftReg<16, true>(u);
 #pragma unroll
@@ -444,7 +490,7 @@ __device__ __attribute__((always_inline)) inline void fastWindow(const FftArgs &
                 }
                 waveFence();
                 {   // inverse step 1: outputs x = j + 16 m of rows 2q (real part) and 2q + 1 (imaginary)
-                    const int j = l;
+                    const int j = li;
                     float2 v[N2];
 #pragma unroll
                     for (int k2b = 0; k2b < N2; k2b++) v[k2b] = pr[17 * k2b + j];
@@ -459,14 +505,12 @@ __device__ __attribute__((always_inline)) inline void fastWindow(const FftArgs &
                                 const int u = 2 * qp + r;
                                 if (r == 1 && !hasB) continue;
                                 const float c = r == 0 ? v[m].x : v[m].y;
-                                const int o = u * bw + x;
-                                const int64_t bs = box[o], bs2 = box2[o];
-                                const int64_t var = int64_t(n) * bs2 - bs * bs;
+                                const float sg = sig[u * bw + x];
                                 float val = 0.0f;
                                 {
 #pragma clang fp contract(off)
                                     const float cc = c * scale;
-                                    if (var > 0 && normk > 0) val = cc / (normk * sqrtf(float(var) / float(n)));
+                                    if (sg > 0 && normk > 0) val = cc / (normk * sg);
                                 }
                                 (r == 0 ? surfA : surfB)[x] = val;
                                 bestOf(best, ib, val, u * M + x);
@@ -609,10 +653,12 @@ __global__ void __launch_bounds__(F::threads) fastReference(FftArgs a)
     }
 }
 
-constexpr int kFastWaves = 8;
+// waves per block: the L / 8 batches of four lines (rows or columns) spread evenly over the four SIMDs
+// (L = 64: 8 waves, one batch each; 96: 12, one each; 128 - 192: 16, one or two each, the same per SIMD)
+constexpr int fastWaves(int L) { return L <= 64 ? 8 : L <= 96 ? 12 : 16; }
 
 template <int L>
-using FastL = Fast<L, kFastWaves>;
+using FastL = Fast<L, fastWaves(L)>;
 
 template <class F>
 void configureOnce()

@@ -9,11 +9,12 @@ namespace framecor {
 
 // the (h + 1) x (w + 1) int64 tables of n frames (n x h x w uint8), pixels and squares
 void launchSummedArea(const uint8_t *frames, int n, int h, int w, int64_t *sat, int64_t *sat2, hipStream_t stream);
-// the W x W box sums of n frames (n x h x w uint8), n x (h - W + 1) x (w - W + 1) int32 each, straight
-// from the pixels: box (may be null) and box2, the sums of the pixels and of their squares, and (screen
-// not null) {box, 1 / sqrt(W^2 box2 - box^2) as float bits (0 where that is not positive)}
+// the W x W box maps of n frames (n x h x w uint8), n x (h - W + 1) x (w - W + 1) each, straight from the
+// pixels; each output may be null: box and box2, the int32 sums of the pixels and of their squares;
+// screen, {box, 1 / sqrt(W^2 box2 - box^2) as float bits (0 where that is not positive)}; sigma, the
+// float sqrtf(float(W^2 box2 - box^2) / float(W^2)) (0 where that is not positive)
 void launchBoxSums(const uint8_t *frames, int n, int h, int w, int W, int32_t *box, int32_t *box2, int2 *screen,
-                   hipStream_t stream);
+                   float *sigma, hipStream_t stream);
 
 // ---- the int8-MFMA form (int8_corr.hip)
 

@@ -3,7 +3,8 @@
 // harnesses that forbid hipMalloc)
 //
 // The one buffer a correlate call needs beyond the caller's inputs and outputs and the correlator's own
-// tables is the box-sum maps of a group of frames (int8: 12 bytes, FFT: 8 bytes per window position).
+// tables is the box-sum maps of a group of frames (int8: 12 bytes, FFT: 4 bytes per window position;
+// the FFT form's workspace for spectra beyond LDS is not taken here: such correlators are refused).
 // Here they live in `scratch`, a file-scope device array of kScratchBytes (64 MiB) that every GPU the
 // library runs on holds until the process ends. Its device address is resolved once per device when a
 // correlator is created on it (framecor_create calls scratchSetup with that device current) and kept

@@ -14,7 +14,9 @@
 //               W^2 * 65025 < 2^31 for W <= 181), and optionally a packed map of
 //               {sum F, 1 / sqrt(W^2 sum F^2 - (sum F)^2) as FP32 (0 for a flat box)} that every window
 //               touching that position shares (one 8-byte load per surface point in the int8 form's
-//               screen). The correlation kernels read these maps instead of eight int64 table entries;
+//               screen), or the FFT form's map of sqrtf(float(W^2 sum F^2 - (sum F)^2) / float(W^2))
+//               (0 for a flat box): the FP32 deviation that cxx/surface.h:30-35 evaluates per surface
+//               point, here once per position and frame. The correlation kernels read these maps instead of eight int64 table entries;
 //               the maps equal the four-term differences of the tables (cxx/surface.h:21-25) exactly.
 
 #include "common.h"
@@ -116,7 +118,8 @@ satCols(int h, int w, int64_t satSize, int64_t *sat, int64_t *sat2)
 constexpr int kBoxThreads = 512, kBoxRows = 64;
 
 __global__ void __launch_bounds__(kBoxThreads)
-boxMaps(const uint8_t *frames, int h, int w, int W, int64_t boxSize, int32_t *box, int32_t *box2, int2 *screen)
+boxMaps(const uint8_t *frames, int h, int w, int W, int64_t boxSize, int32_t *box, int32_t *box2, int2 *screen,
+        float *sigma)
 {
     __shared__ uint32_t pre[2][2][kBoxThreads];
     __shared__ uint32_t tot[2][2][kBoxThreads / kWave];
@@ -175,11 +178,10 @@ boxMaps(const uint8_t *frames, int h, int w, int W, int64_t boxSize, int32_t *bo
             const int32_t bs = int32_t(pre[b][0][tid + W - 1] - lo), bs2 = int32_t(pre[b][1][tid + W - 1] - lo2);
             const int64_t o = f * boxSize + int64_t(y) * bw + x;
             if (box) box[o] = bs;
-            box2[o] = bs2;
-            if (screen) {
-                const int64_t v = n * bs2 - int64_t(bs) * bs;
-                screen[o] = make_int2(bs, __float_as_int(v > 0 ? float(1.0 / sqrt(double(v))) : 0.0f));
-            }
+            if (box2) box2[o] = bs2;
+            const int64_t v = n * bs2 - int64_t(bs) * bs;
+            if (screen) screen[o] = make_int2(bs, __float_as_int(v > 0 ? float(1.0 / sqrt(double(v))) : 0.0f));
+            if (sigma) sigma[o] = v > 0 ? sqrtf(float(v) / float(n)) : 0.0f;
         }
     }
 }
@@ -198,14 +200,14 @@ void launchSummedArea(const uint8_t *frames, int n, int h, int w, int64_t *sat, 
 }
 
 void launchBoxSums(const uint8_t *frames, int n, int h, int w, int W, int32_t *box, int32_t *box2, int2 *screen,
-                   hipStream_t stream)
+                   float *sigma, hipStream_t stream)
 {
     if (n <= 0) return;
     const int bh = h - W + 1, bw = w - W + 1, tx = kBoxThreads - W + 1;
     if (bh <= 0 || bw <= 0 || W > 181) throw std::invalid_argument("framecor: box sums need 1 <= W <= 181 and W <= h, w");
     const int64_t boxSize = int64_t(bh) * bw;
     hipLaunchKernelGGL(boxMaps, dim3((bw + tx - 1) / tx, (bh + kBoxRows - 1) / kBoxRows, n), dim3(kBoxThreads), 0,
-                       stream, frames, h, w, W, boxSize, box, box2, screen);
+                       stream, frames, h, w, W, boxSize, box, box2, screen, sigma);
     FC_HIP_CHECK(hipGetLastError());
 }
 

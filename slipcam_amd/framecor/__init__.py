@@ -347,7 +347,10 @@ class Batch:
 
     clips: a sequence of dicts with height, width, down, across, window, search, frames (per call) and
     optionally stat_size (21). method: 'int8' or 'fft'. tune: time the candidate plans of every tile
-    shape on the first call and keep the fastest (otherwise the automatic plan).
+    shape on the first call and keep the fastest (otherwise the automatic plan). plan_cache: a JSON file
+    of tuned plans by (method, GPU, tile shape), read here and extended after a call that tuned (default:
+    $FRAMECOR_PLAN_CACHE or ~/.cache/slipcam_amd/framecor_plans.json; None: no cache), so a later batch
+    with the same tile shapes on the same kind of GPU starts with their plans and does not tune them.
 
       batch = framecor.Batch(clips, method='int8')
       for c, ref in enumerate(refs):
@@ -355,7 +358,7 @@ class Batch:
       results = batch.correlate(frames)      # frames[c]: frames x height x width uint8 per clip
     """
 
-    def __init__(self, clips, *, method='int8', device=0, tune=True):
+    def __init__(self, clips, *, method='int8', device=0, tune=True, plan_cache=''):
         self._b = None
         m = _METHODS.get(method)
         if m is None:
@@ -370,8 +373,10 @@ class Batch:
             arr[i] = _Clip(int(c['height']), int(c['width']), down.ctypes.data, len(down), across.ctypes.data,
                            len(across), int(c['window']), int(c['search']), int(c.get('stat_size', 21)),
                            int(c['frames']))
+            W, R = int(c['window']), int(c['search'])
+            key = (f'W{W}R{R}S{min(int(c.get("stat_size", 21)), 2 * R + 1)}' if method == 'int8' else f'L{W + 2 * R}')
             self._clips.append(dict(height=int(c['height']), width=int(c['width']), shape=(len(down), len(across)),
-                                    frames=int(c['frames'])))
+                                    frames=int(c['frames']), key=key))
         handle = ctypes.c_void_p()
         _check(_lib.framecor_batch_create(ctypes.cast(arr, ctypes.c_void_p), len(clips), m, int(device),
                                           ctypes.byref(handle)))
@@ -380,6 +385,63 @@ class Batch:
         self._device = int(device)
         self._method = method
         _check(_lib.framecor_batch_set_tuning(self._b, int(bool(tune))))
+        self._cache = None
+        self.plans_from_cache = 0
+        if tune and plan_cache is not None:
+            self._cache = plan_cache or os.environ.get('FRAMECOR_PLAN_CACHE') or os.path.join(
+                os.path.expanduser('~'), '.cache', 'slipcam_amd', 'framecor_plans.json')
+            self._read_plans()
+
+    def _device_key(self):
+        import torch
+        props = torch.cuda.get_device_properties(self._device)
+        return f'{self._method}|{getattr(props, "gcnArchName", props.name)}|{props.multi_processor_count}'
+
+    def _read_plans(self):
+        """The cached plans of this batch's tile shapes (an unreadable cache is ignored)"""
+        import json
+        try:
+            with open(self._cache) as fh:
+                plans = json.load(fh).get(self._device_key(), {})
+        except (OSError, ValueError, AttributeError):
+            return
+        done = set()
+        for c, info in enumerate(self._clips):
+            p = plans.get(info['key'])
+            if p is None or info['key'] in done:
+                continue
+            try:
+                self.set_plan(c, p['variant'], int(p['frames_per_item']), int(p['blocks_per_cu']))
+            except (KeyError, TypeError, ValueError):
+                continue
+            done.add(info['key'])
+        self.plans_from_cache = len(done)
+
+    def _write_plans(self):
+        """Add this batch's tuned plans to the cache (written whole to a temporary file, then renamed)"""
+        import json
+        try:
+            with open(self._cache) as fh:
+                cache = json.load(fh)
+        except (OSError, ValueError):
+            cache = {}
+        mine = cache.setdefault(self._device_key(), {})
+        changed = False
+        for c, info in enumerate(self._clips):
+            p = self.plan(c)
+            if p['tuned'] and mine.get(info['key']) != {k: p[k] for k in ('variant', 'frames_per_item', 'blocks_per_cu')}:
+                mine[info['key']] = {k: p[k] for k in ('variant', 'frames_per_item', 'blocks_per_cu')}
+                changed = True
+        if not changed:
+            return
+        try:
+            os.makedirs(os.path.dirname(os.path.abspath(self._cache)), exist_ok=True)
+            tmp = f'{self._cache}.{os.getpid()}.tmp'
+            with open(tmp, 'w') as fh:
+                json.dump(cache, fh, indent=1, sort_keys=True)
+            os.replace(tmp, self._cache)
+        except OSError:
+            pass
 
     def __del__(self):
         b, destroy = getattr(self, '_b', None), getattr(self, '_destroy', None)
@@ -482,8 +544,11 @@ class Batch:
         arrays = [P(*fr), P(*gr) if gross is not None else None] + [P(*[o[k].data_ptr() for o in outs])
                                                                       for k in ('offsets', 'peak', 'snr', 'cov')]
         stream = torch.cuda.current_stream(dev).cuda_stream
+        untuned = self._cache is not None and not all(self.plan(c)['tuned'] for c in range(len(self._clips)))
         _check(_lib.framecor_batch_correlate_device(self._b, *[None if a is None else ctypes.cast(a, ctypes.c_void_p)
                                                                for a in arrays], stream))
+        if untuned:   # this call tuned the shapes without a plan
+            self._write_plans()
         self._inflight = frames   # the caller's tensors stay alive while the launch may read them
         if host:
             torch.cuda.current_stream(dev).synchronize()

@@ -7,6 +7,8 @@ gross offsets, over several stages of frames, over more clips than one queue lau
 tuning, and on replays of the captured graph.
 """
 
+import os
+
 import numpy as np
 import pytest
 
@@ -59,7 +61,7 @@ def same(a, b, where=''):
 
 
 def run_batch(specs, vids, method, tune=False, gross=None, stage_frames=None):
-    batch = framecor.Batch(specs, method=method, tune=tune)
+    batch = framecor.Batch(specs, method=method, tune=tune, plan_cache=None)
     for c, (ref, _) in enumerate(vids):
         batch.set_reference(c, ref)
     if stage_frames:
@@ -198,3 +200,26 @@ def test_batching_workload_16_clips(gpu, method):
             for k in g2:   # the windows the oracle skipped: its fill values
                 g2[k][:, ~keep] = want[k][:, ~keep]
             assert_fft_parity(g2, want, s['search'], surf=surf)
+
+
+def test_plan_cache(gpu, tmp_path):
+    """Tuned plans persist: a second batch of the same tile shapes reads them from the cache file, runs
+    without tuning and gives the first batch's bits"""
+    specs = every_shape(frames=(2, 4), seed=61, resolutions=((480, 640),))[:3]
+    vids = videos(specs)
+    path = str(tmp_path / 'plans.json')
+    first = framecor.Batch(specs, method='int8', tune=True, plan_cache=path)
+    assert first.plans_from_cache == 0 and not os.path.exists(path)
+    for c, (ref, _) in enumerate(vids):
+        first.set_reference(c, ref)
+    out1 = [{k: v.cpu().numpy() for k, v in o.items()} for o in first.correlate([fr for _, fr in vids])]
+    assert os.path.exists(path)
+    plans = [first.plan(c) for c in range(len(specs))]
+    second = framecor.Batch(specs, method='int8', tune=True, plan_cache=path)
+    assert second.plans_from_cache == len(specs)
+    assert [second.plan(c) for c in range(len(specs))] == plans
+    for c, (ref, _) in enumerate(vids):
+        second.set_reference(c, ref)
+    out2 = second.correlate([fr for _, fr in vids])
+    for c in range(len(specs)):
+        same(out2[c], out1[c], f'clip {c}')

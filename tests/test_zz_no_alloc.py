@@ -36,20 +36,20 @@ def call(entry, cor, frames, gross):
 
 @pytest.fixture(scope='module')
 def video():
-    d = synthetic.fault_video(n=14)
+    d = synthetic.fault_video(n=24)
     return d, torch.from_numpy(d['frames']).cuda()
 
 
 @pytest.mark.parametrize('method', ['int8', 'fft'])
 def test_noalloc_equals_the_device_entry_point(gpu, video, method):
-    """720 x 1280, window 64, search 48: 14 then 5 frames (int8: groups of 6 frames in the 64 MiB array;
-    fft: groups of 10), with per-window gross offsets on the first call"""
+    """720 x 1280, window 64, search 48: 24 then 5 frames (int8: groups of 6 frames in the 64 MiB array;
+    fft: groups of 21), with per-window gross offsets on the first call"""
     d, dev = video
     down, across = framecor.window_grid(720, 1280, 64, 48, 16)
     cor = framecor.Correlator(720, 1280, down, across, window=64, search=48, method=method)
     cor.set_reference(d['ref'])
     ny, nx = cor.shape
-    gross = torch.tensor(synthetic.gross_offsets(14, ny, nx, -8, 8), device='cuda')
+    gross = torch.tensor(synthetic.gross_offsets(24, ny, nx, -8, 8), device='cuda')
     lib = framecor._lib
     for frames, g in ((dev, gross), (dev[3:8].contiguous(), None)):
         want = call(lib.framecor_correlate_device, cor, frames, g)
@@ -69,3 +69,15 @@ def test_noalloc_refuses_a_frame_larger_than_its_array(gpu):
                                                              out.data_ptr(), out.data_ptr(), out.data_ptr(), None)
     assert status == 1
     assert 'exceed the 64 MiB scratch' in framecor._lib.framecor_last_error().decode()
+
+
+def test_noalloc_refuses_fft_spectra_beyond_lds(gpu):
+    """The FFT method with a search window whose spectrum does not fit in LDS (window 128, search 64: 256)
+    needs the correlator's workspace: refused with a clear error"""
+    cor = framecor.Correlator(600, 600, [100], [100], window=128, search=64, method='fft')
+    cor.set_reference(np.random.default_rng(1).integers(0, 256, (600, 600)).astype(np.uint8))
+    frames = torch.zeros((1, 600, 600), dtype=torch.uint8, device='cuda')
+    out = torch.empty((1, 1, 1, 4), dtype=torch.float32, device='cuda')
+    status = framecor._lib.framecor_correlate_device_noalloc(cor._c, frames.data_ptr(), 1, None, out.data_ptr(),
+                                                             out.data_ptr(), out.data_ptr(), out.data_ptr(), None)
+    assert status == 1 and 'workspace' in framecor._lib.framecor_last_error().decode()
