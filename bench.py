@@ -40,8 +40,10 @@ FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector (packed), TFLOP/s
 # HBM traffic per launch of the two correlation kernels on this workload at 64 frames per launch, in KiB
 # per dispatch as rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE read them (one pass each), and the factors
 # that turn them into bytes: what the same counters read for a plain 256 MiB device copy
-# (profiles/counters_int8Correlate_fftCorrelate_720p_64frames.md holds the passes and the commands)
-TRAFFIC_KIB = {'int8': (1.706e6, 4828.0), 'fft': (2.974e6, 7640.0)}
+# (profiles/counters_int8Correlate_fastCorrelate_720p_64frames.md holds the passes, the commands and the
+# commit they were measured on; measure again whenever a kernel changes)
+TRAFFIC_KIB = {'int8': (1.706e6, 4827.0), 'fft': (1.677e6, 136800.0)}
+TRAFFIC_SOURCE = 'profiles/counters_int8Correlate_fastCorrelate_720p_64frames.md (commit effb8bc)'
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 TRAFFIC_FRAMES = 64
 
@@ -56,6 +58,19 @@ def traffic_bytes(method, frames):
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform's processor string"""
+    try:
+        with open('/proc/cpuinfo') as fh:
+            for line in fh:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or 'unknown'
 
 
 def time_method(torch, framecor, d, dev, method, steps, warmup):
@@ -269,7 +284,7 @@ def cpu_baseline(d, seconds, gpu=None):
     elapsed = time.perf_counter() - t0
     nwin = len(down) * len(across)
     line = dict(value=round(nwin * done / elapsed, 1), unit='windows/s', cores=oracle.threads(), kind='port',
-                ms_per_frame=round(elapsed / done * 1e3, 2),
+                cpu_model=cpu_model(), ms_per_frame=round(elapsed / done * 1e3, 2),
                 sample=f'{done} frames x {nwin} windows of the same workload, the oracle restatement of the '
                        f'reference CPU algorithm (own FP32 FFT, no FFTW), OpenMP over windows')
     if gpu:
@@ -283,13 +298,14 @@ def cpu_baseline(d, seconds, gpu=None):
             acc['fft_vs_reference_algorithm']['integer_peak_mismatches'] = int(
                 (np.rint(g['offsets'][both]) != np.rint(ref_path['offsets'][both])).any(-1).sum())
         if 'int8' in gpu:
-            n8 = min(16, len(d['frames']))
-            sample = np.random.default_rng(11).choice(nwin, 48, replace=False)
-            want = o.correlate_direct(d['frames'][:n8], windows=sample)
-            g = {k: v[:n8].reshape((n8, nwin) + v.shape[3:])[:, sample] for k, v in gpu['int8'].items()}
-            wnt = {k: v.reshape((n8, nwin) + v.shape[3:])[:, sample] for k, v in want.items()}
+            # frames before, during and after the slip ramp (frames 20-40), 96 windows
+            fsel = [f for f in (0, 16, 22, 26, 30, 34, 40, 48, 63) if f < len(d['frames'])]
+            sample = np.random.default_rng(11).choice(nwin, 96, replace=False)
+            want = o.correlate_direct(d['frames'][fsel], windows=sample)
+            g = {k: v[fsel].reshape((len(fsel), nwin) + v.shape[3:])[:, sample] for k, v in gpu['int8'].items()}
+            wnt = {k: v.reshape((len(fsel), nwin) + v.shape[3:])[:, sample] for k, v in want.items()}
             acc['int8_vs_fp64_oracle'] = _agreement(g, wnt, np.ones(g['peak'].shape, bool))
-            acc['int8_vs_fp64_oracle']['sample'] = f'48 windows x {n8} frames'
+            acc['int8_vs_fp64_oracle']['sample'] = f'96 windows x frames {fsel}'
         line['accuracy'] = acc
     return line
 
@@ -399,7 +415,8 @@ def main():
                              'unit': 'TFLOP/s', 'frac': round(fft_tflops / FP32_PEAK_TFLOPS, 4),
                              'flops_per_launch': fft_flops_frame * ff['frames_per_launch'],
                              'hbm_gbs': round(fft_gbs, 1), 'hbm_frac': round(fft_gbs / HBM_PEAK_GBS, 4),
-                             'traffic': traffic_bytes('fft', ff['frames_per_launch']),
+                             'traffic': traffic_bytes('fft', ff['frames_per_launch']), 'traffic_source': TRAFFIC_SOURCE,
+                             'kernel': 'fastCorrelate<160>',
                              'algorithmic_bytes': fft_bytes_launch},
                 'set_reference_ms': round(ff['set_reference_ms'], 1),
                 'host_api_ms_per_frame': round(ff['host_api_ms_per_frame'], 4)},
@@ -409,7 +426,7 @@ def main():
         'host_api_ms_per_frame': round(i8['host_api_ms_per_frame'], 4),
         'roofline': {'bound': 'mfma', 'achieved': round(achieved, 1), 'peak': INT8_PEAK_TOPS, 'unit': 'TOP/s',
                      'frac': round(achieved / INT8_PEAK_TOPS, 4),
-                     'traffic': traffic_bytes('int8', i8['frames_per_launch']),
+                     'traffic': traffic_bytes('int8', i8['frames_per_launch']), 'traffic_source': TRAFFIC_SOURCE,
                      'algorithmic_bytes': i8_bytes_launch,
                      'kernel': 'int8Correlate', 'kernel_ms_per_launch': round(i8['kernel_ms_per_launch'], 4),
                      'ops_per_launch': ops_launch},
