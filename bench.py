@@ -268,7 +268,8 @@ def cpu_baseline(d, seconds, gpu=None):
     """The oracle restatement of the reference CPU algorithm timed on a bounded sample of the workload;
     gpu: the GPU forms' results on the same frames ({'fft': dict, 'int8': dict}), which this leg also
     checks -- the FFT form against the restatement (the reference's own path) on the timed frames, the
-    int8 form against the FP64 direct-NCC oracle on 48 sampled windows of 16 frames"""
+    int8 form against the FP64 direct-NCC oracle on 96 sampled windows of 9 frames before, during and
+    after the slip ramp"""
     import oracle
     from slipcam_amd import framecor
     h, w = d['frames'].shape[1:]

@@ -6,8 +6,9 @@ Parity of both GPU forms over the range of geometries they support, against the 
          the FP64 direct-NCC oracle, with gross offsets that push some search windows out of the frame
   fft    search windows whose size has factors the radix-2/4/8 passes do not cover (3, 5, 7, 11, the
          primes 29, 37, 67 and 197: direct passes), the largest ones held in LDS (192, 197) and larger ones
-         whose spectra go to the global workspace (201, 256), against the restatement of the reference
-         algorithm; a size above 2048 is refused
+         whose spectra go to the global workspace (201, 256), every size of the register-resident
+         kernel (64, 96, 128, 160, 192), against the restatement of the reference algorithm; a size
+         above 2048 is refused
   both   every window invalid; a flat reference
 """
 
@@ -57,9 +58,10 @@ def test_int8_shapes_vs_fp64_oracle(gpu, W, R):
 
 # L = 29, 35, 44, 77, 96, 192 (the largest power-of-two-and-3 size in LDS), 37 and 197 (primes: direct
 # passes; 197 the largest odd size in LDS), 256 (window 128 and 160: spectra in the global workspace),
-# 201 = 3 x 67 (global workspace and a direct pass of 67)
+# 201 = 3 x 67 (global workspace and a direct pass of 67); 64, 96, 128, 160 and 192 are the sizes of the
+# register-resident kernel
 FFT_SHAPES = [(15, 7), (21, 7), (24, 10), (33, 22), (16, 40), (64, 64), (25, 6), (97, 50), (128, 64), (160, 48),
-              (101, 50)]
+              (101, 50), (32, 16), (64, 32), (80, 40)]
 
 
 @pytest.mark.parametrize('W,R', FFT_SHAPES)
