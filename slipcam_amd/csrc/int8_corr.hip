@@ -28,413 +28,11 @@
 // the 3 x 3 FP64 fit and the covariance on exact FP64 values (cxx/surface.h:71-146 formulas).
 
 #include "common.h"
-#include "epilogue.h"
-#include "kernels.h"
+#include "int8_device.h"
 
 namespace framecor {
 
 namespace {
-
-typedef int v4i __attribute__((ext_vector_type(4)));
-
-constexpr int kPipedWaves = 12;   // producers + helpers per block of the pipelined kernel
-
-struct WaveBest {
-    double value;
-    int index;
-    int pad;
-};
-
-struct Moments {
-    int64_t n;      // W^2
-    int64_t tvar;   // W^2 sum T^2 - (sum T)^2
-    double nd;      // W^2
-    double tau;     // sum (T - 128) = sum T - 128 W^2
-    float rst;      // 1 / sqrt(tvar)
-};
-
-// the exact numerator W^2 sum(FT) - sum T sum F = W^2 r - tau phi, phi = sum F - 128 W^2: an integer
-// below 2^53, so the FP64 product tau phi and the fused multiply-add are exact
-__device__ inline double numerator(int raw, int fs, const Moments &m)
-{
-    const double phi = double(fs - 128 * int(m.n));
-    return fma(m.nd, double(raw), -(m.tau * phi));
-}
-
-// the exact NCC in FP64 (the same value as the direct oracle's); 0 where the search box (or the
-// template) is flat, as cxx/surface.h:30-35
-__device__ inline double exactNcc(int raw, int fs, int fs2, const Moments &m)
-{
-    const int64_t vf = m.n * int64_t(fs2) - int64_t(fs) * int64_t(fs);
-    if (vf <= 0 || m.tvar <= 0) return 0.0;
-    return numerator(raw, fs, m) / sqrt(double(m.tvar) * double(vf));
-}
-
-// the same to FP32 accuracy (a relative error of a few 1e-7), with the frame's 1 / sqrt(vf) map
-__device__ inline float screenNcc(int raw, int fs, float rsig, const Moments &m)
-{
-    return float(numerator(raw, fs, m)) * rsig * m.rst;
-}
-
-// a lane's two largest screened values, the index (u M + v) and raw value of the largest; equal values:
-// the lower index
-struct Top2 {
-    float f1, f2;
-    int i1, r1;
-};
-
-__device__ inline Top2 top2Init() { return Top2{-3.0f, -3.0f, 0x7fffffff, 0}; }
-
-// A lane adds its points in ascending index order, so of equal values the first stays the largest and
-// only a strictly larger value replaces it. Branch-free: the second largest is the larger of the old
-// second and the smaller of (x, the old largest).
-__device__ inline void top2Add(Top2 &t, float x, int idx, int raw)
-{
-    const bool win = x > t.f1;
-    t.f2 = fmaxf(t.f2, fminf(x, t.f1));
-    t.f1 = fmaxf(t.f1, x);
-    t.i1 = win ? idx : t.i1;
-    t.r1 = win ? raw : t.r1;
-}
-
-// the raw int32 surface of one 16-row band (rows 16 band .. + 15, all NVT v tiles) of a staged chip
-template <int NVT, int NJB>
-__device__ inline void bandMfma(v4i (&acc)[NVT], const int8_t *chip, const int8_t *tmpl, int band, int lane, int ni4,
-                                int tstride, int cstride)
-{
-    const int g4 = lane >> 4, m16 = lane & 15;
-    // B fragment geometry: lane (g4, n = m16) takes template row 4 i4 + g4, bytes 16 + 16 jb - n .. + 15
-    // of the zero-padded LDS row, i.e. dwords (16 - n) / 4 + 4 jb .. + 4, funnel-shifted
-    const int bbyte = 16 - m16, bdw = bbyte >> 2, bsh = bbyte & 3;
-#pragma unroll
-    for (int v1 = 0; v1 < NVT; v1++) acc[v1] = v4i{0, 0, 0, 0};
-    const int8_t *crow = chip + (16 * band + g4 + m16) * cstride;
-    const uint32_t *trow = reinterpret_cast<const uint32_t *>(tmpl + g4 * tstride) + bdw;
-    const int cstep = 4 * cstride, tstep = tstride;   // 4 rows: bytes; dwords per 4 rows
-    for (int i4 = 0; i4 < ni4; i4++) {
-        v4i b[NJB];
-#pragma unroll
-        for (int jb = 0; jb < NJB; jb++) {
-            const uint32_t *t = trow + 4 * jb;
-            const uint32_t w0 = t[0], w1 = t[1], w2 = t[2], w3 = t[3], w4 = t[4];
-            b[jb] = v4i{int(__builtin_amdgcn_alignbyte(w1, w0, bsh)), int(__builtin_amdgcn_alignbyte(w2, w1, bsh)),
-                        int(__builtin_amdgcn_alignbyte(w3, w2, bsh)), int(__builtin_amdgcn_alignbyte(w4, w3, bsh))};
-        }
-#pragma unroll
-        for (int cb = 0; cb < NVT + NJB - 1; cb++) {
-            const v4i av = *reinterpret_cast<const v4i *>(crow + 16 * cb);
-#pragma unroll
-            for (int jb = 0; jb < NJB; jb++) {
-                const int v1 = cb - jb;
-                if (v1 >= 0 && v1 < NVT) acc[v1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b[jb], acc[v1], 0, 0, 0);
-            }
-        }
-        crow += cstep;
-        trow += tstep;
-    }
-}
-
-// The edge row and column of a surface of M = 16 c + 1 points (every search range R = 8 c):
-// the c x c core tiles come from c bands of c tiles; the row u = 16 c and the column v = 16 c are
-// computed here instead of as a 17th band and a 17th tile column (16 rows or columns computed to keep
-// one). Every LDS read is unconditional (lanes whose outputs are discarded read clamped addresses), so
-// the loads of an iteration are issued together ahead of its MFMAs.
-//
-// The edge row: the v blocks (0..c) on the MFMA's M axis, the template Toeplitz on N as in the bands:
-//   A[m][(i, jb, kk)] = F[16 c + i][16 (m + jb) + kk],  B = the bands' fragments,  C[m][n] = r[16 c][16 m + n]
-// (rows m > c of C are discarded; their lanes read the chip at m = c).
-template <int NJB>
-__device__ inline v4i edgeRowMfma(const int8_t *chip, const int8_t *tmpl, int c, int lane, int ni4, int tstride,
-                                  int cstride)
-{
-    v4i acc = v4i{0, 0, 0, 0};   // one chain: MFMAs feeding their own accumulator issue back to back
-    const int g4 = lane >> 4, m16 = lane & 15;
-    const int bbyte = 16 - m16, bdw = bbyte >> 2, bsh = bbyte & 3;
-    const int8_t *crow = chip + (16 * c + g4) * cstride + 16 * min(m16, c);
-    const uint32_t *trow = reinterpret_cast<const uint32_t *>(tmpl + g4 * tstride) + bdw;
-    for (int i4 = 0; i4 < ni4; i4++) {
-        v4i av[NJB], b[NJB];
-#pragma unroll
-        for (int jb = 0; jb < NJB; jb++) {
-            const uint32_t *t = trow + 4 * jb;
-            const uint32_t w0 = t[0], w1 = t[1], w2 = t[2], w3 = t[3], w4 = t[4];
-            b[jb] = v4i{int(__builtin_amdgcn_alignbyte(w1, w0, bsh)), int(__builtin_amdgcn_alignbyte(w2, w1, bsh)),
-                        int(__builtin_amdgcn_alignbyte(w3, w2, bsh)), int(__builtin_amdgcn_alignbyte(w4, w3, bsh))};
-            av[jb] = *reinterpret_cast<const v4i *>(crow + 16 * jb);
-        }
-#pragma unroll
-        for (int jb = 0; jb < NJB; jb++) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[jb], b[jb], acc, 0, 0, 0);
-        crow += 4 * cstride;
-        trow += tstride;
-    }
-    return acc;
-}
-
-// The edge column: a Toeplitz expansion of the template along its columns' transpose, the u blocks
-// (0..c-1) on M and the 16 rows of a block on N:
-//   A[m][(i', jb, kk)] = F[16 m + i'][16 c + 16 jb + kk],  B[(i', jb, kk)][n] = T[i' - n][16 jb + kk],
-//   C[m][n] = sum over i' in [0, W + 15), jb in [0, ceil(W / 16))  -> r[16 m + n][16 c]
-// i' is taken four at a time (i' = 4 q + g4, one per 16-lane group), jb inner (immediate LDS offsets).
-// Rows m >= c of C are discarded (their lanes read the chip at m = c - 1); a B fragment outside the
-// template (i' - n outside 0..W-1) reads the zero block (16 NJB zero bytes).
-template <int NJB>
-__device__ inline v4i edgeColMfma(const int8_t *chip, const int8_t *tmpl, const int8_t *zero, int c, int W, int lane,
-                                  int tstride, int cstride)
-{
-    v4i acc = v4i{0, 0, 0, 0};
-    const int g4 = lane >> 4, m16 = lane & 15;
-    const int nJ = (W + 15) >> 4, nq = (W + 18) >> 2;
-    const int8_t *abase = chip + 16 * min(m16, c - 1) * cstride + 16 * c;
-    for (int q = 0; q < nq; q++) {
-        const int ip = 4 * q + g4, ti = ip - m16;
-        const bool inside = ip < W + 15 && ti >= 0 && ti < W;
-        const int8_t *arow = abase + (ip < W + 15 ? ip : 0) * cstride;
-        const int8_t *brow = inside ? tmpl + ti * tstride + 16 : zero;
-        v4i av[NJB], bv[NJB];
-#pragma unroll
-        for (int jb = 0; jb < NJB; jb++)
-            if (jb < nJ) {
-                av[jb] = *reinterpret_cast<const v4i *>(arow + 16 * jb);
-                bv[jb] = *reinterpret_cast<const v4i *>(brow + 16 * jb);
-            }
-#pragma unroll
-        for (int jb = 0; jb < NJB; jb++)
-            if (jb < nJ) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[jb], bv[jb], acc, 0, 0, 0);
-    }
-    return acc;
-}
-
-// the chip of a frame (L x L at (sy, sx)) as int8 (pixels ^ 0x80 = pixels - 128) into LDS, by threads
-// t0 .. t0 + nt - 1 of the block (t: this thread's index among them)
-__device__ inline void loadChip(int8_t *chip, const uint8_t *src, int w, int L, int cstride, int t, int nt)
-{
-    // thread t takes the 16-byte column chunk t % cpr of rows t / cpr, t / cpr + nt / cpr, ... (cpr <= nt)
-    const int cpr = L >> 4;
-    if (cpr > 0) {
-        const int rowsPer = nt / cpr, r0 = t / cpr, cc = t - r0 * cpr;
-        if (r0 < rowsPer) {
-            const uint8_t *s = src + size_t(r0) * w + 16 * cc;
-            int8_t *d = chip + r0 * cstride + 16 * cc;
-            const size_t sstep = size_t(rowsPer) * w;
-            const int dstep = rowsPer * cstride;
-            for (int r = r0; r < L; r += rowsPer) {
-                uint4 v;
-                __builtin_memcpy(&v, s, 16);
-                v.x ^= 0x80808080u;
-                v.y ^= 0x80808080u;
-                v.z ^= 0x80808080u;
-                v.w ^= 0x80808080u;
-                *reinterpret_cast<uint4 *>(d) = v;
-                s += sstep;
-                d += dstep;
-            }
-        }
-    }
-    const int tail = L & 15;
-    if (tail)
-        for (int c = t; c < L * tail; c += nt) {
-            const int r = c / tail, j = 16 * cpr + (c - r * tail);
-            chip[r * cstride + j] = int8_t(src[size_t(r) * w + j] ^ 0x80);
-        }
-}
-
-// zero the template and chip areas (the padding stays zero), then the template as int8 rows at byte 16
-__device__ inline void loadTemplate(unsigned char *lds, int zeroBytes, int8_t *tmpl, const uint8_t *T, int W,
-                                    int tstride, int tid, int nthreads)
-{
-    for (int i = tid * 16; i < zeroBytes; i += nthreads * 16) *reinterpret_cast<v4i *>(lds + i) = v4i{0, 0, 0, 0};
-    __syncthreads();
-    for (int idx = tid; idx < W * W; idx += nthreads) {
-        const int i = idx / W, j = idx - i * W;
-        tmpl[i * tstride + 16 + j] = int8_t(T[idx] ^ 0x80);
-    }
-}
-
-// the window k of frame f: its gross offset, search window and validity (uniform over the block)
-struct FrameWindow {
-    int64_t q;
-    int gy, gx, sy, sx;
-    bool valid;
-};
-
-__device__ inline FrameWindow frameWindow(const Int8Args &a, int k, int f, int y0, int x0, float normk)
-{
-    FrameWindow fw;
-    fw.q = int64_t(f) * a.g.nwin + k;
-    fw.gy = 0;
-    fw.gx = 0;
-    if (a.fg.gross) {
-        fw.gy = a.fg.gross[2 * fw.q];
-        fw.gx = a.fg.gross[2 * fw.q + 1];
-    }
-    fw.valid = normk > 0 && searchWindow(a.g, y0, x0, fw.gy, fw.gx, fw.sy, fw.sx);
-    return fw;
-}
-
-// floor(x / d) for 0 <= x < 2^20, 1 <= d < 2^12 by a multiply-high with m = ceil(2^32 / d) (d > 1)
-__device__ inline int quot(int x, unsigned m, int d) { return d == 1 ? x : int(__umulhi(unsigned(x), m)); }
-__device__ inline unsigned quotMagic(int d) { return unsigned((0x100000000ull + d - 1) / unsigned(d)); }
-
-constexpr int kBatch = 8;   // surface points per lane whose loads are in flight together
-
-// helper wave h of nh: the rows h, h + nh, ... of the raw surface S of frame f: the FP32 screen, then
-// the exact FP64 best among the points within the screen's error of the wave's maximum, into *out
-__device__ inline void helperScreen(const Int8Args &a, const Moments &mom, const int *S, const FrameWindow &fw, int f,
-                                    int h, int nh, int lane, WaveBest *out)
-{
-    const Geometry &g = a.g;
-    const int M = g.M;
-    const int64_t boff = f * g.boxSize + int64_t(fw.sy) * g.bw + fw.sx;
-    const int2 *scr = a.fg.screen + boff;
-    const int32_t *box2 = a.fg.box2 + boff;
-    const int nrows = (M - h + nh - 1) / nh, total = nrows * M;
-    const unsigned mM = quotMagic(M);
-    // pass 1: the two largest screened values of each lane, with the index of the largest
-    float f1 = -3.0f, f2 = -3.0f;
-    int i1 = 0x7fffffff;
-    for (int e0 = lane; e0 < total; e0 += kBatch * kWave) {
-        int raw[kBatch], fs[kBatch], at[kBatch];
-        float rs[kBatch];
-#pragma unroll
-        for (int t = 0; t < kBatch; t++) {
-            const int e = e0 + t * kWave;
-            at[t] = -1;
-            if (e < total) {
-                const int r = quot(e, mM, M), v = e - r * M, u = h + r * nh;
-                at[t] = u * M + v;
-                raw[t] = S[at[t]];
-                const int2 p = scr[u * g.bw + v];
-                fs[t] = p.x;
-                rs[t] = __int_as_float(p.y);
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < kBatch; t++)
-            if (at[t] >= 0) {
-                const float x = screenNcc(raw[t], fs[t], rs[t], mom);
-                if (x > f1 || (x == f1 && at[t] < i1)) {
-                    f2 = f1;
-                    f1 = x;
-                    i1 = at[t];
-                }
-                else if (x > f2)
-                    f2 = x;
-            }
-    }
-    float fmax = f1;
-    int imax = i1;
-    waveBest(fmax, imax);
-    const float thr = fmax - 1e-5f * fabsf(fmax) - 1e-30f;
-    double dbest = -3.0;
-    int dibest = 0x7fffffff;
-    if (__any(f2 >= thr)) {
-        // a lane holds two candidates: every point within the screen's error of the maximum
-        for (int e = lane; e < total; e += kWave) {
-            const int r = quot(e, mM, M), v = e - r * M, u = h + r * nh;
-            const int2 p = scr[u * g.bw + v];
-            const int raw = S[u * M + v];
-            if (screenNcc(raw, p.x, __int_as_float(p.y), mom) >= thr)
-                bestOf(dbest, dibest, exactNcc(raw, p.x, box2[u * g.bw + v], mom), u * M + v);
-        }
-    }
-    else if (f1 >= thr) {
-        const int u = i1 / M, v = i1 - u * M, o = u * g.bw + v;
-        dbest = exactNcc(S[i1], scr[o].x, box2[o], mom);
-        dibest = i1;
-    }
-    waveBest(dbest, dibest);
-    if (lane == 0) {
-        out->value = dbest;
-        out->index = dibest;
-    }
-}
-
-// The SNR box around the peak (ub, vb) of an M x M surface: n = min(stat, M) points a side, from
-// (peak - n / 2), clipped to the surface (cxx/surface.h:108-124)
-struct SnrBox {
-    int ub, vb, ulo, uhi, vlo, vhi, bwid, items;
-    bool inside;   // the peak has its 3 x 3 neighbours
-};
-
-__device__ inline SnrBox snrBox(int ib, int M, int stat)
-{
-    SnrBox b;
-    b.ub = ib / M;
-    b.vb = ib - b.ub * M;
-    const int nn = stat < M ? stat : M;
-    b.ulo = max(b.ub - nn / 2, 0);
-    b.uhi = min(b.ub - nn / 2 + nn, M);
-    b.vlo = max(b.vb - nn / 2, 0);
-    b.vhi = min(b.vb - nn / 2 + nn, M);
-    b.bwid = b.vhi - b.vlo;
-    b.items = (b.uhi - b.ulo) * b.bwid;
-    b.inside = b.ub > 0 && b.ub < M - 1 && b.vb > 0 && b.vb < M - 1;
-    return b;
-}
-
-// one wave: the outputs of frame f from its exact peak (best, ib): the SNR over the stat box and the 3 x 3
-// FP64 fit and covariance on exact values (cxx/surface.h:71-146 formulas). The raw surface S is the whole
-// M x M surface, or (boxed) only the SNR box, row-major from its corner.
-__device__ inline void finishPeak(const Int8Args &a, const Moments &mom, const int *S, bool boxed, const FrameWindow &fw,
-                                  int f, double best, int ib, int lane)
-{
-    const Geometry &g = a.g;
-    const int M = g.M;
-    if (!fw.valid || !(best > 0)) {
-        if (lane == 0) invalidateOut(fw.q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
-        return;
-    }
-    const int64_t boff = f * g.boxSize + int64_t(fw.sy) * g.bw + fw.sx;
-    const int2 *scr = a.fg.screen + boff;
-    const int32_t *box2 = a.fg.box2 + boff;
-    const SnrBox b = snrBox(ib, M, g.stat);
-    const int sStride = boxed ? b.bwid : M, su = boxed ? b.ulo : 0, sv = boxed ? b.vlo : 0;
-    double sum = 0.0;
-    // the stat box, kBatch points per lane in flight; lanes 0..8 also take the 3 x 3 neighbours
-    double mine = 0.0;
-    for (int it0 = lane; it0 < b.items; it0 += kBatch * kWave) {
-        int raw[kBatch], fs[kBatch], f2v[kBatch];
-        bool ok[kBatch];
-#pragma unroll
-        for (int t = 0; t < kBatch; t++) {
-            const int it = it0 + t * kWave;
-            ok[t] = it < b.items;
-            if (ok[t]) {
-                const int u = b.ulo + it / b.bwid, v = b.vlo + it % b.bwid, o = u * g.bw + v;
-                raw[t] = S[(u - su) * sStride + v - sv];
-                fs[t] = scr[o].x;
-                f2v[t] = box2[o];
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < kBatch; t++)
-            if (ok[t]) {
-                const double d = exactNcc(raw[t], fs[t], f2v[t], mom);
-                sum += d * d;
-            }
-    }
-    if (b.inside && lane < 9) {
-        const int u = b.ub + lane / 3 - 1, v = b.vb + lane % 3 - 1, o = u * g.bw + v;
-        mine = exactNcc(S[(u - su) * sStride + v - sv], scr[o].x, box2[o], mom);
-    }
-    sum = waveSumD(sum);
-    double fsq[3][3];
-    for (int i = 0; i < 9; i++) fsq[i / 3][i % 3] = __shfl(mine, i, kWave);
-    fsq[1][1] = best;
-    if (lane == 0)
-        writePeakF64(fsq, b.inside, b.ub, b.vb, g.R, fw.gy, fw.gx, sum, b.items, mom.nd, fw.q, a.fg.offset, a.fg.peak,
-                     a.fg.snr, a.fg.cov);
-}
-
-// one wave: the peak of frame f from the helpers' results over the whole raw surface S
-__device__ inline void helperFinish(const Int8Args &a, const Moments &mom, const int *S, const FrameWindow &fw, int f,
-                                    const WaveBest *part, int nh, int lane)
-{
-    double best = -3.0;
-    int ib = 0x7fffffff;
-    for (int t = 0; t < nh; t++) bestOf(best, ib, part[t].value, part[t].index);
-    finishPeak(a, mom, S, false, fw, f, best, ib, lane);
-}
 
 // Window k against frames fBegin .. fEnd - 1, by the whole block (NVT + EW waves).
 // EW = 0: ceil(M / 16) = NVT full bands; EW = 2 (M = 16 NVT + 1): NVT core bands plus a wave for the edge
@@ -1029,12 +627,40 @@ Int8Layout int8Layout(const Geometry &g)
     l.pipedShmem = size_t(4 * l.ni4) * l.tstride + 2 * size_t(l.crows) * l.cstride
         + 2 * sizeof(int) * size_t((g.M * g.M + 3) & ~3) + 2 * sizeof(WaveBest) * size_t(nh > 0 ? nh : 0);
     l.pipedOk = l.supported && nh >= 2 && l.pipedShmem <= 160 * 1024;
+    // the 32 x 32 kernel: template rows with 32 zero bytes in front (the 32 x 32 fragments read bytes
+    // 32 - n .. 32 + 16 nj + 3, the 16 x 16 edge fragments 16 bytes in), at a stride whose two rows of a
+    // fragment build (nine dwords each) fall on disjoint banks; chip rows of an odd number of 16-byte units
+    l.nb = (g.M - 1) / 32;
+    l.nj = (g.W + 46) / 16;
+    l.wideOk = l.supported && g.M % 32 == 1 && l.nb >= 1 && l.nb <= 4 && int8WideKernel(l.nb, l.nj, l.njb, false);
+    if (l.wideOk) {
+        int ts = std::max(32 + 16 * l.nj + 4, 36 + 16 * l.njb);
+        ts = (ts + 15) & ~15;
+        while ((ts / 4) % 64 < 12 || (ts / 4) % 64 > 52) ts += 16;
+        l.wideTstride = ts;
+        int wcs = std::max(std::max(16 * (2 * l.nb + l.nj - 2), 16 * (2 * l.nb + l.njb)), g.L);
+        wcs = (wcs + 15) / 16;
+        if (wcs % 2 == 0) wcs++;
+        l.wideCstride = 16 * wcs;
+        l.wideCrows = 32 * l.nb + 4 * l.ni4;
+        l.wideShmem = size_t(int8Lds(l.ni4, l.wideTstride, l.wideCrows, l.wideCstride, l.njb, g.M, g.stat, l.nb + 2).total);
+        l.wideOk = l.wideShmem <= 160 * 1024;
+    }
     return l;
 }
 
 Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue)
 {
     if (!l.supported) throw std::invalid_argument("framecor: the int8 form supports windows up to 129 pixels and search ranges up to 71 pixels");
+    if (l.wideOk && variant == Int8Variant::Wide) {
+        Int8Kernel k{};
+        k.func = int8WideKernel(l.nb, l.nj, l.njb, queue);
+        k.threads = kWave * (l.nb + 2);
+        k.shmem = l.wideShmem;
+        k.variant = Int8Variant::Wide;
+        allowLargeLds(k.func);
+        return k;
+    }
     const bool piped = variant == Int8Variant::Piped && l.pipedOk;
     const bool edge = !piped && variant != Int8Variant::Padded && l.edge;
     const int kind = piped ? 2 : edge ? 1 : 0, nvt = edge ? l.nvt - 1 : l.nvt;
@@ -1054,13 +680,14 @@ Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue)
     return k;
 }
 
-void fillInt8Args(Int8Args &a, const Int8Layout &l)
+void fillInt8Args(Int8Args &a, const Int8Layout &l, Int8Variant variant)
 {
-    a.nut = l.nut;
+    const bool wide = variant == Int8Variant::Wide && l.wideOk;
+    a.nut = wide ? l.nb : l.nut;
     a.ni4 = l.ni4;
-    a.tstride = l.tstride;
-    a.cstride = l.cstride;
-    a.crows = l.crows;
+    a.tstride = wide ? l.wideTstride : l.tstride;
+    a.cstride = wide ? l.wideCstride : l.cstride;
+    a.crows = wide ? l.wideCrows : l.crows;
     a.alias = l.alias ? 1 : 0;
 }
 
@@ -1068,7 +695,7 @@ void launchInt8(Int8Args a, const Int8Layout &l, Int8Variant variant, hipStream_
 {
     if (a.fg.n <= 0) return;
     const Int8Kernel k = int8Kernel(l, variant, false);
-    fillInt8Args(a, l);
+    fillInt8Args(a, l, k.variant);
     // frames per block: at most 16 (at 720p, window 64, search 48, 64 frames in blocks of 16 run 3 %
     // faster than in blocks of 64, and the batching autotuner settles on 16 for most tile shapes), and
     // enough blocks to fill the GPU several times over

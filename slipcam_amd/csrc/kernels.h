@@ -58,12 +58,17 @@ struct Int8Layout {
     bool edge;            // M = 16 c + 1: c core bands plus a wave for the edge row and one for the edge column
     size_t pipedShmem;    // the pipelined producer/helper kernel
     bool pipedOk;
+    // the 32 x 32 kernel (int8_wide.hip) where M = 32 nb + 1: nb bands, nj template blocks, its LDS
+    bool wideOk;
+    int nb, nj, wideTstride, wideCstride, wideCrows;
+    size_t wideShmem;
 };
 
 // Auto and Simple: the one-frame-at-a-time kernel (two blocks per CU), with the edge row and the edge
 // column in a wave each where M = 16 c + 1; Piped: the producer/helper pipeline where its buffers fit
-// in LDS; Padded: the one-frame kernel with ceil(M / 16) full bands (no edge waves)
-enum class Int8Variant { Auto = 0, Simple = 1, Piped = 2, Padded = 3 };
+// in LDS; Padded: the one-frame kernel with ceil(M / 16) full bands (no edge waves); Wide: the
+// one-frame kernel on 32 x 32 tiles (v_mfma_i32_32x32x32_i8) where M = 32 c + 1, c <= 4 (else as Auto)
+enum class Int8Variant { Auto = 0, Simple = 1, Piped = 2, Padded = 3, Wide = 4 };
 
 Int8Layout int8Layout(const Geometry &g);
 void launchInt8(Int8Args a, const Int8Layout &l, Int8Variant variant, hipStream_t stream);
@@ -78,8 +83,10 @@ struct Int8Kernel {
 
 // the kernel of a tile shape (l) and variant; queue: the persistent work-queue form
 Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue);
-// the layout fields of the arguments (nut, ni4, strides, alias)
-void fillInt8Args(Int8Args &a, const Int8Layout &l);
+// the layout fields of the arguments (nut, ni4, strides, alias) for the kernel variant (resolved)
+void fillInt8Args(Int8Args &a, const Int8Layout &l, Int8Variant variant);
+// the 32 x 32 kernel of nb bands and nj (njb) template blocks; null where none is instantiated
+const void *int8WideKernel(int nb, int nj, int njb, bool queue);
 // the persistent form over a work queue; clips: device array of the clips' arguments (layout filled)
 void launchInt8Queue(const Int8Kernel &k, const WorkQueue &q, const Int8Args *clips, int blocks, hipStream_t stream);
 

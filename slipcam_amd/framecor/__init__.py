@@ -198,10 +198,11 @@ class Correlator(_framecor.Correlator):
         super().set_reference(frame)
 
     def set_variant(self, variant):
-        """The int8 kernel: 'auto', 'simple' (one frame at a time), 'piped' (producer/helper pipeline) or
-        'padded' (one frame at a time, full 16-row bands even where only an edge row and column remain);
-        every variant gives the same offsets, peaks and covariances, the SNR to rounding"""
-        _check(_lib.framecor_set_variant(self._c, {'auto': 0, 'simple': 1, 'piped': 2, 'padded': 3}[variant]))
+        """The int8 kernel: 'auto' (as 'simple'), 'simple' (one frame at a time, 16 x 16 tiles), 'piped' (producer/helper pipeline), 'padded' (one
+        frame at a time, full 16-row bands even where only an edge row and column remain) or 'wide' (one
+        frame at a time, 32 x 32 tiles, search ranges 16, 32, 48, 64; else as 'auto'); every variant gives
+        the same offsets, peaks and covariances, the SNR to rounding"""
+        _check(_lib.framecor_set_variant(self._c, _VARIANTS[variant]))
 
     def set_timing(self, enable=True):
         """Record the GPU time of each stage of the next correlate calls (hipEvents on their stream)"""
@@ -337,7 +338,7 @@ class _Clip(ctypes.Structure):
                 ('search', ctypes.c_int), ('stat_size', ctypes.c_int), ('frames', ctypes.c_int)]
 
 
-_VARIANTS = {'auto': 0, 'simple': 1, 'piped': 2, 'padded': 3}
+_VARIANTS = {'auto': 0, 'simple': 1, 'piped': 2, 'padded': 3, 'wide': 4}
 
 
 class Batch:

@@ -36,8 +36,10 @@ def workload(W, R, n=2, seed=0):
     return dict(ref=synthetic.to_uint8(base), frames=frames, down=down, across=across, gross=gross, h=h, w=w)
 
 
+# the 32 x 32 kernel (search 16, 32, 48, 64) at every template width class it instantiates:
+# (3, 16), (20, 16), (40, 48), (64, 16/32), (80, 32), (90, 16), (100, 48), (125, 64), (128, 64)
 INT8_SHAPES = [(3, 1), (16, 8), (20, 5), (33, 17), (48, 24), (40, 56), (64, 16), (64, 32), (96, 40), (128, 64),
-               (129, 71)]
+               (129, 71), (3, 16), (20, 16), (40, 48), (80, 32), (90, 16), (100, 48), (125, 64)]
 
 
 @pytest.mark.parametrize('W,R', INT8_SHAPES)
@@ -50,7 +52,7 @@ def test_int8_shapes_vs_fp64_oracle(gpu, W, R):
     assert valid.any()
     cor = framecor.Correlator(d['h'], d['w'], d['down'], d['across'], window=W, search=R, method='int8')
     cor.set_reference(d['ref'])
-    for variant in ('auto', 'padded', 'piped'):
+    for variant in ('simple', 'padded', 'piped', 'wide'):
         cor.set_variant(variant)
         got = cor.correlate(d['frames'], gross=d['gross'])
         assert_int8_parity(got, ref)
