@@ -74,6 +74,10 @@ struct framecor_batch {
     hipEvent_t lastLaunch = nullptr;
     bool launched = false;
     DeviceBuffer dArgs, dCounters;
+    // the pointers of the call whose arguments are in dArgs (the autotuner rewrites entries of stage 0)
+    const uint8_t *const *argFr = nullptr;
+    const int32_t *const *argGross = nullptr;
+    float *const *argOut[4] = {nullptr, nullptr, nullptr, nullptr};
     int counterSets = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -164,7 +168,9 @@ struct framecor_batch {
         return fg;
     }
 
-    Int8Args int8Args(int c, const FrameGroup &fg) const
+    // the arguments of clip c for the kernel variant of its shape (or the given variant: the layout
+    // fields differ between the 16 x 16 and 32 x 32 kernels)
+    Int8Args int8Args(int c, const FrameGroup &fg, int variant = -1) const
     {
         const framecor_correlator &k = *clips[c];
         Int8Args a{};
@@ -173,7 +179,8 @@ struct framecor_batch {
         a.fg = fg;
         a.tmpl = k.dTmpl.as<uint8_t>();
         const Shape &sh = shapes[shapeOf[c]];
-        fillInt8Args(a, sh.layout, int8Kernel(sh.layout, Int8Variant(sh.variant), true).variant);
+        const int v = variant >= 0 ? variant : sh.variant;
+        fillInt8Args(a, sh.layout, int8Kernel(sh.layout, Int8Variant(v), true).variant);
         return a;
     }
 
@@ -245,6 +252,12 @@ struct framecor_batch {
     void writeArgs(const uint8_t *const *fr, const int32_t *const *gross, float *const *offset, float *const *peak,
                    float *const *snr, float *const *cov)
     {
+        argFr = fr;
+        argGross = gross;
+        argOut[0] = offset;
+        argOut[1] = peak;
+        argOut[2] = snr;
+        argOut[3] = cov;
         const int n = int(clips.size());
         std::vector<unsigned char> host(argSize() * size_t(stages) * n);
         for (int s = 0; s < stages; s++)
@@ -262,6 +275,22 @@ struct framecor_batch {
             }
         dArgs.ensure(host.size());
         FC_HIP_CHECK(hipMemcpy(dArgs.ptr, host.data(), host.size(), hipMemcpyHostToDevice));
+    }
+
+    // the stage-0 arguments of the clips of shape sh for the int8 kernel variant under trial (the table
+    // must not be in use: the autotuner waits for each trial)
+    void writeTrialArgs(const Shape &sh, int variant)
+    {
+        if (!isInt8()) return;
+        const int n = int(clips.size());
+        for (int c : sh.clips) {
+            const FrameGroup fg = stageGroup(c, 0, argFr[c], argGross ? argGross[c] : nullptr, argOut[0][c], argOut[1][c],
+                                             argOut[2][c], argOut[3][c]);
+            const Int8Args a = int8Args(c, fg, variant);
+            FC_HIP_CHECK(hipMemcpy(static_cast<unsigned char *>(dArgs.ptr) + argSize() * size_t(c), &a, sizeof(a),
+                                   hipMemcpyHostToDevice));
+        }
+        (void)n;
     }
 
     const void *stageArgs(int s) const
@@ -347,11 +376,17 @@ struct framecor_batch {
                 }
                 float best = 1e30f;
                 Shape keep = sh;
+                int lastTrial = -1;
                 for (int v : variants)
                     for (int fpb : {1, 2, 4, 8, 16})
                         for (int perCu : {1, 2, 4}) {
                             Shape cand = sh;
                             cand.variant = v;
+                            if (cand.variant != lastTrial) {
+                                // the previous trial's kernels are over (each trial waits for its events)
+                                writeTrialArgs(cand, cand.variant);
+                                lastTrial = cand.variant;
+                            }
                             cand.fpb = fpb;
                             cand.perCu = perCu;
                             // grids beyond the occupancy are the launch of a smaller perCu
@@ -420,7 +455,11 @@ struct framecor_batch {
             // the frames come from the caller's stream
             FC_HIP_CHECK(hipEventRecord(lastLaunch, user));
             FC_HIP_CHECK(hipStreamWaitEvent(stream, lastLaunch, 0));
-            if (needTune) autotune(fr);
+            if (needTune) {
+                autotune(fr);
+                // the tuned variants' layout fields
+                writeArgs(fr, gross, offset, peak, snr, cov);
+            }
             prepareCall();
             FC_HIP_CHECK(hipStreamSynchronize(stream));
             FC_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));

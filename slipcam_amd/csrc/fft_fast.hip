@@ -342,7 +342,8 @@ __device__ inline void packedColumnTask(float2 *S, const float2 *tw, const float
     waveFence();
     if (k2 < N2) {
         const float2 *pv = spec + size_t(k2) * 16, *qv = spec + (size_t(H) * N2 + k2) * 16;
-#pragma unroll 4
+        // fully unrolled: u stays in registers (a partly unrolled loop indexes it at run time, in scratch)
+#pragma unroll
         for (int k1 = 0; k1 < 16; k1++) {
             const float2 zc = S[F::idx(partnerRow<F>(k2, k1), 0)];
             u[k1] = cadd(cmul(u[k1], pv[k1]), cmul(make_float2(zc.x, -zc.y), qv[k1]));
