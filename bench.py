@@ -42,8 +42,8 @@ FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector (packed), TFLOP/s
 # that turn them into bytes: what the same counters read for a plain 256 MiB device copy
 # (profiles/counters_int8Correlate_fastCorrelate_720p_64frames.md holds the passes, the commands and the
 # commit they were measured on; measure again whenever a kernel changes)
-TRAFFIC_KIB = {'int8': (1.706e6, 4827.0), 'fft': (1.677e6, 136800.0)}
-TRAFFIC_SOURCE = 'profiles/counters_int8Correlate_fastCorrelate_720p_64frames.md (commit effb8bc)'
+TRAFFIC_KIB = {'int8': (1.706e6, 4828.0), 'fft': (1.676e6, 84670.0)}
+TRAFFIC_SOURCE = 'profiles/counters_int8Correlate_fastCorrelate_720p_64frames.md (commit 9f12ae0)'
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 TRAFFIC_FRAMES = 64
 
