@@ -194,8 +194,10 @@ int framecor_batch_plan(const framecor_batch *b, int clip, int *plan);
 /* sets the plan of clip's tile shape (plan[0..2] as above) */
 int framecor_batch_set_plan(framecor_batch *b, int clip, const int *plan);
 
-/* at most frames (>= 1) frames of each clip per stage (the default: what 2 GiB of box-sum maps per clip
- * hold, at most 256); a call runs the stages one after the other, each its box sums then its kernels */
+/* at most frames (>= 1) frames of each clip per stage (the default: every frame in one stage where the
+ * box-sum maps of all clips fit in a quarter of the free device memory, at most 64 GiB; else what 2 GiB
+ * of box-sum maps per clip hold, at most 256); a call runs the stages one after the other, each one
+ * launch of the box sums of every clip, then the work-queue kernels */
 int framecor_batch_set_stage_frames(framecor_batch *b, int frames);
 
 /* info[0] tile shapes, info[1] stages of frames, info[2] nodes of the captured graph (0 before the first

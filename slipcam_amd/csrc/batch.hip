@@ -74,6 +74,9 @@ struct framecor_batch {
     hipEvent_t lastLaunch = nullptr;
     bool launched = false;
     DeviceBuffer dArgs, dCounters;
+    // the box-map jobs of each stage ([stage][clip], the first jobCount[stage] in use) and their blocks
+    DeviceBuffer dJobs;
+    std::vector<int> jobCount, jobBlocks;
     // the pointers of the call whose arguments are in dArgs (the autotuner rewrites entries of stage 0)
     const uint8_t *const *argFr = nullptr;
     const int32_t *const *argGross = nullptr;
@@ -275,6 +278,37 @@ struct framecor_batch {
             }
         dArgs.ensure(host.size());
         FC_HIP_CHECK(hipMemcpy(dArgs.ptr, host.data(), host.size(), hipMemcpyHostToDevice));
+        // the box maps of each stage: one job per clip with frames in it
+        std::vector<BoxJob> jobs(size_t(stages) * n);
+        jobCount.assign(stages, 0);
+        jobBlocks.assign(stages, 0);
+        for (int st = 0; st < stages; st++)
+            for (int c = 0; c < n; c++) {
+                const framecor_correlator &k = *clips[c];
+                int first, count;
+                stageFrames(c, st, first, count);
+                if (count <= 0) continue;
+                BoxJob &j = jobs[size_t(st) * n + jobCount[st]++];
+                j.frames = fr[c] + size_t(first) * k.g.pixels;
+                j.box = nullptr;
+                j.box2 = isInt8() ? k.dBox2.as<int32_t>() : nullptr;
+                j.screen = isInt8() ? k.dScreen.as<int2>() : nullptr;
+                j.sigma = isInt8() ? nullptr : k.dSigma.as<float>();
+                j.n = count;
+                j.h = k.g.h;
+                j.w = k.g.w;
+                j.W = k.g.W;
+                j.first = jobBlocks[st];
+                jobBlocks[st] += boxJobBlocks(count, k.g.h, k.g.w, k.g.W);
+            }
+        dJobs.ensure(sizeof(BoxJob) * jobs.size());
+        FC_HIP_CHECK(hipMemcpy(dJobs.ptr, jobs.data(), sizeof(BoxJob) * jobs.size(), hipMemcpyHostToDevice));
+    }
+
+    // the box maps of every clip's frames of stage st: one launch
+    void launchStageBoxes(int st, hipStream_t s) const
+    {
+        launchBoxJobs(dJobs.as<BoxJob>() + size_t(st) * clips.size(), jobCount[st], jobBlocks[st], s);
     }
 
     // the stage-0 arguments of the clips of shape sh for the int8 kernel variant under trial (the table
@@ -310,7 +344,7 @@ struct framecor_batch {
     }
 
     // the launch sequence of one call (captured into the graph)
-    void enqueueCall(const uint8_t *const *fr, hipStream_t s) const
+    void enqueueCall(hipStream_t s) const
     {
         const int sets = counterSets;
         int *counters = dCounters.as<int>();
@@ -318,15 +352,7 @@ struct framecor_batch {
         FC_HIP_CHECK(hipGetLastError());
         int set = 0;
         for (int st = 0; st < stages; st++) {
-            for (size_t c = 0; c < clips.size(); c++) {
-                framecor_correlator &k = *clips[c];
-                int first, count;
-                stageFrames(int(c), st, first, count);
-                if (count <= 0) continue;
-                launchBoxSums(fr[c] + size_t(first) * k.g.pixels, count, k.g.h, k.g.w, k.g.W, nullptr,
-                              isInt8() ? k.dBox2.as<int32_t>() : nullptr, isInt8() ? k.dScreen.as<int2>() : nullptr,
-                              isInt8() ? nullptr : k.dSigma.as<float>(), s);
-            }
+            launchStageBoxes(st, s);
             for (const Shape &sh : shapes)
                 for (WorkQueue q : queues(sh, st)) {
                     q.counters = counters + kXcds * set++;
@@ -336,7 +362,7 @@ struct framecor_batch {
     }
 
     // times the candidates of every shape without a plan on samples of stage 0 and keeps the fastest
-    void autotune(const uint8_t *const *fr)
+    void autotune()
     {
         hipEvent_t e0, e1;
         FC_HIP_CHECK(hipEventCreate(&e0));
@@ -347,15 +373,7 @@ struct framecor_batch {
         counters = dTune.as<int>();
         try {
             // the box sums of stage 0 (the samples read them)
-            for (size_t c = 0; c < clips.size(); c++) {
-                framecor_correlator &k = *clips[c];
-                int first, count;
-                stageFrames(int(c), 0, first, count);
-                if (count > 0)
-                    launchBoxSums(fr[c], count, k.g.h, k.g.w, k.g.W, nullptr, isInt8() ? k.dBox2.as<int32_t>() : nullptr,
-                                  isInt8() ? k.dScreen.as<int2>() : nullptr, isInt8() ? nullptr : k.dSigma.as<float>(),
-                                  stream);
-            }
+            launchStageBoxes(0, stream);
             for (Shape &sh : shapes) {
                 if (sh.tuned) continue;
                 // a sample of about 64 items per CU, and at least as many frames as the largest frames per
@@ -456,7 +474,7 @@ struct framecor_batch {
             FC_HIP_CHECK(hipEventRecord(lastLaunch, user));
             FC_HIP_CHECK(hipStreamWaitEvent(stream, lastLaunch, 0));
             if (needTune) {
-                autotune(fr);
+                autotune();
                 // the tuned variants' layout fields
                 writeArgs(fr, gross, offset, peak, snr, cov);
             }
@@ -464,7 +482,7 @@ struct framecor_batch {
             FC_HIP_CHECK(hipStreamSynchronize(stream));
             FC_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
             try {
-                enqueueCall(fr, stream);
+                enqueueCall(stream);
             }
             catch (...) {
                 hipGraph_t g = nullptr;
@@ -514,11 +532,21 @@ int framecor_batch_create(const framecor_clip *clips, int nclips, int method, in
             auto cor = std::make_unique<framecor_correlator>();
             cor->init(k.height, k.width, k.down, k.ndown, k.across, k.nacross, k.window, k.search, method, 1,
                       k.stat_size, device);
-            if (k.frames > 0) cor->ensureGroup(k.frames);
             b->clips.push_back(std::move(cor));
             b->frames.push_back(k.frames);
         }
         FC_HIP_CHECK(hipSetDevice(device));
+        // the box maps of every clip's frames in one stage where they fit in a quarter of the free device
+        // memory (at most 64 GiB); else each clip's frames in groups of its own limit (2 GiB of maps)
+        size_t mapBytes = 0, freeBytes = 0, totalBytes = 0;
+        for (int c = 0; c < nclips; c++) mapBytes += size_t(clips[c].frames) * b->clips[c]->bytesPerFrame();
+        FC_HIP_CHECK(hipMemGetInfo(&freeBytes, &totalBytes));
+        const bool oneStage = mapBytes <= std::min(freeBytes / 4, size_t(64) << 30);
+        for (int c = 0; c < nclips; c++) {
+            framecor_correlator &k = *b->clips[c];
+            if (oneStage) k.maxGroup = std::max(k.maxGroup, clips[c].frames);
+            if (clips[c].frames > 0) k.ensureGroup(clips[c].frames);
+        }
         FC_HIP_CHECK(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
         FC_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
         FC_HIP_CHECK(hipEventCreateWithFlags(&b->lastLaunch, hipEventDisableTiming));

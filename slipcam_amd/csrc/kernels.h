@@ -16,6 +16,20 @@ void launchSummedArea(const uint8_t *frames, int n, int h, int w, int64_t *sat, 
 void launchBoxSums(const uint8_t *frames, int n, int h, int w, int W, int32_t *box, int32_t *box2, int2 *screen,
                    float *sigma, hipStream_t stream);
 
+// the box maps of n frames of one clip (as launchBoxSums) as one job of a launch over several clips;
+// first: the job's first block in the launch (the jobs in order, each boxJobBlocks(n, h, w, W) blocks)
+struct BoxJob {
+    const uint8_t *frames;
+    int32_t *box, *box2;
+    int2 *screen;
+    float *sigma;
+    int n, h, w, W;
+    int first;
+};
+int boxJobBlocks(int n, int h, int w, int W);
+// jobs: a device array of njobs jobs; blocks: the sum of their blocks
+void launchBoxJobs(const BoxJob *jobs, int njobs, int blocks, hipStream_t stream);
+
 // ---- the int8-MFMA form (int8_corr.hip)
 
 struct Int8Args {

@@ -117,15 +117,16 @@ satCols(int h, int w, int64_t satSize, int64_t *sat, int64_t *sat2)
 // exact. Two LDS buffers (by row parity) let one row's prefixes be read while the next row's are made.
 constexpr int kBoxThreads = 512, kBoxRows = 64;
 
-__global__ void __launch_bounds__(kBoxThreads)
-boxMaps(const uint8_t *frames, int h, int w, int W, int64_t boxSize, int32_t *box, int32_t *box2, int2 *screen,
-        float *sigma)
+// block (bx, by) of frame f: output columns bx (kBoxThreads - W + 1) .., rows by kBoxRows ..
+__device__ __attribute__((always_inline)) inline void boxMapsBlock(const uint8_t *frames, int h, int w, int W,
+                                                                   int64_t boxSize, int32_t *box, int32_t *box2,
+                                                                   int2 *screen, float *sigma, int bx, int by, int f)
 {
     __shared__ uint32_t pre[2][2][kBoxThreads];
     __shared__ uint32_t tot[2][2][kBoxThreads / kWave];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
     const int bh = h - W + 1, bw = w - W + 1, tx = kBoxThreads - W + 1;
-    const int x0 = blockIdx.x * tx, y0 = blockIdx.y * kBoxRows, f = blockIdx.z;
+    const int x0 = bx * tx, y0 = by * kBoxRows;
     const int x = x0 + tid, yEnd = min(y0 + kBoxRows, bh);
     const bool mine = x < w;
     const uint8_t *F = frames + size_t(f) * h * w + x;
@@ -186,6 +187,26 @@ boxMaps(const uint8_t *frames, int h, int w, int W, int64_t boxSize, int32_t *bo
     }
 }
 
+__global__ void __launch_bounds__(kBoxThreads)
+boxMaps(const uint8_t *frames, int h, int w, int W, int64_t boxSize, int32_t *box, int32_t *box2, int2 *screen,
+        float *sigma)
+{
+    boxMapsBlock(frames, h, w, W, boxSize, box, box2, screen, sigma, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// the box maps of several clips in one launch: block b belongs to the job j with first[j] <= b < first[j + 1]
+__global__ void __launch_bounds__(kBoxThreads) boxMapsJobs(const BoxJob *__restrict__ jobs, int njobs)
+{
+    const int b = blockIdx.x;
+    int j = 0;
+    while (j + 1 < njobs && b >= jobs[j + 1].first) j++;
+    const BoxJob &jb = jobs[j];
+    const int bw = jb.w - jb.W + 1, bh = jb.h - jb.W + 1, tx = kBoxThreads - jb.W + 1;
+    const int gx = (bw + tx - 1) / tx, gy = (bh + kBoxRows - 1) / kBoxRows;
+    const int local = b - jb.first, bx = local % gx, rest = local / gx, by = rest % gy, f = rest / gy;
+    boxMapsBlock(jb.frames, jb.h, jb.w, jb.W, int64_t(bh) * bw, jb.box, jb.box2, jb.screen, jb.sigma, bx, by, f);
+}
+
 } // namespace
 
 void launchSummedArea(const uint8_t *frames, int n, int h, int w, int64_t *sat, int64_t *sat2,
@@ -196,6 +217,20 @@ void launchSummedArea(const uint8_t *frames, int n, int h, int w, int64_t *sat, 
     hipLaunchKernelGGL(satRows, dim3((h + kRowsPerBlock - 1) / kRowsPerBlock, n), dim3(kWave * kRowsPerBlock),
                        0, stream, frames, h, w, satSize, sat, sat2);
     hipLaunchKernelGGL(satCols, dim3((w + 255) / 256, n), dim3(256), 0, stream, h, w, satSize, sat, sat2);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+int boxJobBlocks(int n, int h, int w, int W)
+{
+    const int bh = h - W + 1, bw = w - W + 1, tx = kBoxThreads - W + 1;
+    if (n <= 0 || bh <= 0 || bw <= 0) return 0;
+    return ((bw + tx - 1) / tx) * ((bh + kBoxRows - 1) / kBoxRows) * n;
+}
+
+void launchBoxJobs(const BoxJob *jobs, int njobs, int blocks, hipStream_t stream)
+{
+    if (njobs <= 0 || blocks <= 0) return;
+    hipLaunchKernelGGL(boxMapsJobs, dim3(blocks), dim3(kBoxThreads), 0, stream, jobs, njobs);
     FC_HIP_CHECK(hipGetLastError());
 }
 

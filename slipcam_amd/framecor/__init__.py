@@ -481,7 +481,9 @@ class Batch:
         _check(_lib.framecor_batch_set_plan(self._b, int(clip), p))
 
     def set_stage_frames(self, frames):
-        """At most this many frames of each clip per stage (default: what 2 GiB of box-sum maps hold)"""
+        """At most this many frames of each clip per stage (default: every frame in one stage where the
+        box-sum maps of all clips fit in a quarter of the free device memory, at most 64 GiB; else what
+        2 GiB of a clip's maps hold)"""
         _check(_lib.framecor_batch_set_stage_frames(self._b, int(frames)))
 
     def info(self):
