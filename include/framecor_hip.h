@@ -129,13 +129,15 @@ int framecor_ncc_backward(framecor_correlator *c, const float *frames, int n, co
 int framecor_ncc_refine(framecor_correlator *c, const float *frames, int n, const int32_t *gross,
                         const float *shifts_in, int steps, float *shifts_out, float *ncc, void *stream);
 
-/* the kernel of the int8 method (no counterpart in the reference; a tuning knob): 0 automatic (as 1),
- * 1 one frame at a time on 16 x 16 tiles, 2 the pipelined producer/helper kernel
- * (falls back to 1 where its LDS buffers do not fit), 3 one frame at a time with ceil(M / 16) full
- * 16-row bands (where M = 2 search + 1 = 16 c + 1, variant 1 computes c x c tiles plus the edge row and
- * the edge column in a wave each), 4 one frame at a time on 32 x 32 tiles (v_mfma_i32_32x32x32_i8)
- * where M = 32 c + 1, c <= 4 (search 16, 32, 48, 64; else as 0); the offsets, peaks and covariances
- * are the same, the SNR to rounding */
+/* the kernel of the int8 method (no counterpart in the reference; a tuning knob): 0 automatic (as 5
+ * where it applies, else as 1), 1 one frame at a time on 16 x 16 tiles, 2 the pipelined
+ * producer/helper kernel (falls back to 1 where its LDS buffers do not fit), 3 one frame at a time with
+ * ceil(M / 16) full 16-row bands (where M = 2 search + 1 = 16 c + 1, variant 1 computes c x c tiles plus
+ * the edge row and the edge column in a wave each), 4 one frame at a time on 32 x 32 tiles
+ * (v_mfma_i32_32x32x32_i8) where M = 32 c + 1, c <= 4 (search 16, 32, 48, 64; else as 1), 5 one frame
+ * at a time in blocks of four waves with equal MFMA shares where M = 16 c + 1, c = 4 or 6 (search 32 or
+ * 48) and the window is 32 or 64 (else as 1); the offsets, peaks and covariances are the same, the SNR
+ * to rounding */
 int framecor_set_variant(framecor_correlator *c, int variant);
 
 /* per-stage GPU timing (no counterpart in the reference; for benchmarks): with enable 1, the
@@ -187,7 +189,8 @@ int framecor_batch_set_reference(framecor_batch *b, int clip, const uint8_t *fra
 int framecor_batch_set_tuning(framecor_batch *b, int enable);
 
 /* the plan of clip's tile shape: plan[0] the int8 kernel variant (as framecor_set_variant; resolved:
- * 1 edge waves, 2 pipelined, 3 full bands, 4 32 x 32 tiles), plan[1] frames per work item (0: automatic), plan[2]
+ * 1 edge waves, 2 pipelined, 3 full bands, 4 32 x 32 tiles, 5 four-wave blocks), plan[1] frames per
+ * work item (0: automatic), plan[2]
  * resident blocks per CU (0: as many as fit), plan[3] 1 if tuned or set */
 int framecor_batch_plan(const framecor_batch *b, int clip, int *plan);
 

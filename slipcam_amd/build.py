@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'libframecor_hip.so')
 MODULE_SRC = 'framecor_module.cpp'
 MODULE = os.path.join(HERE, 'framecor', '_framecor' + sysconfig.get_config_var('EXT_SUFFIX'))
-SOURCES = ['capi.hip', 'batch.hip', 'sat.hip', 'int8_corr.hip', 'int8_wide.hip', 'fft_corr.hip', 'fft_fast.hip', 'diff.hip',
+SOURCES = ['capi.hip', 'batch.hip', 'sat.hip', 'int8_corr.hip', 'int8_wide.hip', 'int8_quad.hip', 'fft_corr.hip', 'fft_fast.hip', 'diff.hip',
            'noalloc.hip']
 VERSION = '0.1.0'
 

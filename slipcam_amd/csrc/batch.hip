@@ -384,6 +384,7 @@ struct framecor_batch {
                 const int cap = int(std::max<int64_t>(16, std::min<int64_t>(64, (int64_t(cus) * 64 + perFrame - 1) / perFrame)));
                 std::vector<int> variants;
                 if (isInt8()) {
+                    if (sh.layout.quadOk) variants.push_back(int(Int8Variant::Quad));
                     if (sh.layout.wideOk) variants.push_back(int(Int8Variant::Wide));
                     if (sh.layout.edge) variants.push_back(int(Int8Variant::Simple));
                     variants.push_back(int(Int8Variant::Padded));
@@ -624,8 +625,8 @@ int framecor_batch_set_plan(framecor_batch *b, int clip, const int *plan)
 {
     if (!b || !plan) return fail(FRAMECOR_EINVAL, "framecor: null argument");
     if (clip < 0 || clip >= int(b->clips.size())) return fail(FRAMECOR_EINVAL, "framecor: no clip " + std::to_string(clip));
-    if (plan[0] < 0 || plan[0] > 4 || plan[1] < 0 || plan[2] < 0 || plan[2] > 64)
-        return fail(FRAMECOR_EINVAL, "framecor: a plan is (variant 0..4, frames per item >= 0, blocks per CU 0..64)");
+    if (plan[0] < 0 || plan[0] > 5 || plan[1] < 0 || plan[2] < 0 || plan[2] > 64)
+        return fail(FRAMECOR_EINVAL, "framecor: a plan is (variant 0..5, frames per item >= 0, blocks per CU 0..64)");
     return guarded([&] {
         if (b->launched) FC_HIP_CHECK(hipEventSynchronize(b->lastLaunch));
         Shape &sh = b->shapes[b->shapeOf[clip]];

@@ -603,6 +603,7 @@ int int8JBlocks(int W)
 Int8Layout int8Layout(const Geometry &g)
 {
     Int8Layout l{};
+    l.W = g.W;
     l.njb = int8JBlocks(g.W);
     l.nvt = (g.M + 15) / 16;
     l.nut = l.nvt;
@@ -646,6 +647,29 @@ Int8Layout int8Layout(const Geometry &g)
         l.wideShmem = size_t(int8Lds(l.ni4, l.wideTstride, l.wideCrows, l.wideCstride, l.njb, g.M, g.stat, l.nb + 2).total);
         l.wideOk = l.wideShmem <= 160 * 1024;
     }
+    // the four-wave kernel: the chip's L rows only, and the narrowest template rows (a multiple of 16
+    // bytes, for the edge column's 16-byte reads) whose four rows of a fragment build (five dwords each
+    // at the same offset) fall on disjoint banks; four blocks per CU at window 64, search 48
+    {
+        int ts = (16 * l.njb + 20 + 15) & ~15;
+        for (;; ts += 16) {
+            bool ok = true;
+            for (int r = 1; r < 4 && ok; r++) {
+                const int d = (r * ts / 4) % 64;
+                ok = d >= 5 && d <= 59;
+            }
+            for (int r1 = 1; r1 < 4 && ok; r1++)
+                for (int r2 = r1 + 1; r2 < 4 && ok; r2++) {
+                    const int d = ((r2 - r1) * ts / 4) % 64;
+                    ok = d >= 5 && d <= 59;
+                }
+            if (ok) break;
+        }
+        l.quadTstride = ts;
+    }
+    l.quadCrows = g.L;
+    l.quadShmem = int8QuadShmem(l.ni4, l.quadTstride, l.quadCrows, l.cstride, l.njb, g);
+    l.quadOk = l.supported && l.edge && l.quadShmem <= 80 * 1024 && int8QuadKernel(l.nvt - 1, g.W, l.njb, false);
     return l;
 }
 
@@ -658,6 +682,15 @@ Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue)
         k.threads = kWave * (l.nb + 2);
         k.shmem = l.wideShmem;
         k.variant = Int8Variant::Wide;
+        allowLargeLds(k.func);
+        return k;
+    }
+    if (l.quadOk && (variant == Int8Variant::Quad || variant == Int8Variant::Auto)) {
+        Int8Kernel k{};
+        k.func = int8QuadKernel(l.nvt - 1, l.W, l.njb, queue);
+        k.threads = int8QuadThreads();
+        k.shmem = l.quadShmem;
+        k.variant = Int8Variant::Quad;
         allowLargeLds(k.func);
         return k;
     }
@@ -682,12 +715,12 @@ Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue)
 
 void fillInt8Args(Int8Args &a, const Int8Layout &l, Int8Variant variant)
 {
-    const bool wide = variant == Int8Variant::Wide && l.wideOk;
+    const bool wide = variant == Int8Variant::Wide && l.wideOk, quad = variant == Int8Variant::Quad && l.quadOk;
     a.nut = wide ? l.nb : l.nut;
     a.ni4 = l.ni4;
-    a.tstride = wide ? l.wideTstride : l.tstride;
+    a.tstride = wide ? l.wideTstride : quad ? l.quadTstride : l.tstride;
     a.cstride = wide ? l.wideCstride : l.cstride;
-    a.crows = wide ? l.wideCrows : l.crows;
+    a.crows = wide ? l.wideCrows : quad ? l.quadCrows : l.crows;
     a.alias = l.alias ? 1 : 0;
 }
 

@@ -15,6 +15,7 @@ namespace {
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int kPipedWaves = 12;   // producers + helpers per block of the pipelined kernel
+constexpr int kEdgeChains = 3;    // independent accumulators of the edge row and column waves
 
 struct WaveBest {
     double value;
@@ -124,7 +125,11 @@ template <int NJB>
 __device__ inline v4i edgeRowMfma(const int8_t *chip, const int8_t *tmpl, int c, int lane, int ni4, int tstride,
                                   int cstride)
 {
-    v4i acc = v4i{0, 0, 0, 0};   // one chain: MFMAs feeding their own accumulator issue back to back
+    // kEdgeChains independent MFMA chains (a chain issues an MFMA only once the previous one has
+    // finished), column block jb into chain jb % kEdgeChains, summed exactly in int32 at the end
+    v4i acc[kEdgeChains];
+#pragma unroll
+    for (int k = 0; k < kEdgeChains; k++) acc[k] = v4i{0, 0, 0, 0};
     const int g4 = lane >> 4, m16 = lane & 15;
     const int bbyte = 16 - m16, bdw = bbyte >> 2, bsh = bbyte & 3;
     const int8_t *crow = chip + (16 * c + g4) * cstride + 16 * min(m16, c);
@@ -140,11 +145,14 @@ __device__ inline v4i edgeRowMfma(const int8_t *chip, const int8_t *tmpl, int c,
             av[jb] = *reinterpret_cast<const v4i *>(crow + 16 * jb);
         }
 #pragma unroll
-        for (int jb = 0; jb < NJB; jb++) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[jb], b[jb], acc, 0, 0, 0);
+        for (int jb = 0; jb < NJB; jb++)
+            acc[jb % kEdgeChains] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[jb], b[jb], acc[jb % kEdgeChains], 0, 0, 0);
         crow += 4 * cstride;
         trow += tstride;
     }
-    return acc;
+#pragma unroll
+    for (int k = 1; k < kEdgeChains; k++) acc[0] += acc[k];
+    return acc[0];
 }
 
 // The edge column: a Toeplitz expansion of the template along its columns' transpose, the u blocks
@@ -158,7 +166,9 @@ template <int NJB>
 __device__ inline v4i edgeColMfma(const int8_t *chip, const int8_t *tmpl, const int8_t *zero, int c, int W, int lane,
                                   int tstride, int cstride)
 {
-    v4i acc = v4i{0, 0, 0, 0};
+    v4i acc[kEdgeChains];   // independent chains, as the edge row
+#pragma unroll
+    for (int k = 0; k < kEdgeChains; k++) acc[k] = v4i{0, 0, 0, 0};
     const int g4 = lane >> 4, m16 = lane & 15;
     const int nJ = (W + 15) >> 4, nq = (W + 18) >> 2;
     const int8_t *abase = chip + 16 * min(m16, c - 1) * cstride + 16 * c;
@@ -176,9 +186,13 @@ __device__ inline v4i edgeColMfma(const int8_t *chip, const int8_t *tmpl, const 
             }
 #pragma unroll
         for (int jb = 0; jb < NJB; jb++)
-            if (jb < nJ) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[jb], bv[jb], acc, 0, 0, 0);
+            if (jb < nJ)
+                acc[jb % kEdgeChains] =
+                    __builtin_amdgcn_mfma_i32_16x16x64_i8(av[jb], bv[jb], acc[jb % kEdgeChains], 0, 0, 0);
     }
-    return acc;
+#pragma unroll
+    for (int k = 1; k < kEdgeChains; k++) acc[0] += acc[k];
+    return acc[0];
 }
 
 // the chip of a frame (L x L at (sy, sx)) as int8 (pixels ^ 0x80 = pixels - 128) into LDS, by threads

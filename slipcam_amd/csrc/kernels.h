@@ -65,6 +65,7 @@ __host__ __device__ inline Int8Lds int8Lds(int ni4, int tstride, int crows, int 
 }
 
 struct Int8Layout {
+    int W;                // the template side
     int njb, nvt, nut, ni4, tstride, cstride, crows;
     size_t shmem;         // the one-frame-at-a-time kernel
     bool alias;           // (unused: the one-frame kernel keeps no full surface)
@@ -76,13 +77,20 @@ struct Int8Layout {
     bool wideOk;
     int nb, nj, wideTstride, wideCstride, wideCrows;
     size_t wideShmem;
+    // the four-wave pipelined kernel (int8_quad.hip) where M = 16 c + 1, c = 4 or 6, and W = 32 or 64: the
+    // LDS of its block
+    bool quadOk;
+    int quadTstride, quadCrows;
+    size_t quadShmem;
 };
 
-// Auto and Simple: the one-frame-at-a-time kernel (two blocks per CU), with the edge row and the edge
-// column in a wave each where M = 16 c + 1; Piped: the producer/helper pipeline where its buffers fit
+// Auto: Quad where it applies, else Simple; Simple: the one-frame-at-a-time kernel (two blocks per CU),
+// with the edge row and the edge column in a wave each where M = 16 c + 1; Piped: the producer/helper pipeline where its buffers fit
 // in LDS; Padded: the one-frame kernel with ceil(M / 16) full bands (no edge waves); Wide: the
-// one-frame kernel on 32 x 32 tiles (v_mfma_i32_32x32x32_i8) where M = 32 c + 1, c <= 4 (else as Auto)
-enum class Int8Variant { Auto = 0, Simple = 1, Piped = 2, Padded = 3, Wide = 4 };
+// one-frame kernel on 32 x 32 tiles (v_mfma_i32_32x32x32_i8) where M = 32 c + 1, c <= 4 (else as Auto);
+// Quad: blocks of four waves with equal MFMA shares, software-pipelined over frames, where M = 16 c + 1,
+// c = 4 or 6, and W = 32 or 64 (else as Auto)
+enum class Int8Variant { Auto = 0, Simple = 1, Piped = 2, Padded = 3, Wide = 4, Quad = 5 };
 
 Int8Layout int8Layout(const Geometry &g);
 void launchInt8(Int8Args a, const Int8Layout &l, Int8Variant variant, hipStream_t stream);
@@ -101,6 +109,11 @@ Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue);
 void fillInt8Args(Int8Args &a, const Int8Layout &l, Int8Variant variant);
 // the 32 x 32 kernel of nb bands and nj (njb) template blocks; null where none is instantiated
 const void *int8WideKernel(int nb, int nj, int njb, bool queue);
+// the four-wave kernel of c core bands, a template of W columns in njb blocks; null where none is
+// instantiated; its block size and LDS
+const void *int8QuadKernel(int c, int W, int njb, bool queue);
+int int8QuadThreads();
+size_t int8QuadShmem(int ni4, int tstride, int crows, int cstride, int njb, const Geometry &g);
 // the persistent form over a work queue; clips: device array of the clips' arguments (layout filled)
 void launchInt8Queue(const Int8Kernel &k, const WorkQueue &q, const Int8Args *clips, int blocks, hipStream_t stream);
 

@@ -198,10 +198,12 @@ class Correlator(_framecor.Correlator):
         super().set_reference(frame)
 
     def set_variant(self, variant):
-        """The int8 kernel: 'auto' (as 'simple'), 'simple' (one frame at a time, 16 x 16 tiles), 'piped' (producer/helper pipeline), 'padded' (one
-        frame at a time, full 16-row bands even where only an edge row and column remain) or 'wide' (one
-        frame at a time, 32 x 32 tiles, search ranges 16, 32, 48, 64; else as 'auto'); every variant gives
-        the same offsets, peaks and covariances, the SNR to rounding"""
+        """The int8 kernel: 'auto' (as 'quad' where it applies, else 'simple'), 'simple' (one frame at a
+        time, 16 x 16 tiles), 'piped' (producer/helper pipeline), 'padded' (one frame at a time, full
+        16-row bands even where only an edge row and column remain), 'wide' (one frame at a time, 32 x 32
+        tiles, search ranges 16, 32, 48, 64; else as 'simple') or 'quad' (one frame at a time in blocks of
+        four waves with equal MFMA shares, search ranges 32 and 48 with windows of 32 or 64; else as
+        'simple'); every variant gives the same offsets, peaks and covariances, the SNR to rounding"""
         _check(_lib.framecor_set_variant(self._c, _VARIANTS[variant]))
 
     def set_timing(self, enable=True):
@@ -338,7 +340,7 @@ class _Clip(ctypes.Structure):
                 ('search', ctypes.c_int), ('stat_size', ctypes.c_int), ('frames', ctypes.c_int)]
 
 
-_VARIANTS = {'auto': 0, 'simple': 1, 'piped': 2, 'padded': 3, 'wide': 4}
+_VARIANTS = {'auto': 0, 'simple': 1, 'piped': 2, 'padded': 3, 'wide': 4, 'quad': 5}
 
 
 class Batch:

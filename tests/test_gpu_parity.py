@@ -184,7 +184,7 @@ def test_fault_fft_vs_reference_algorithm(gpu, fault):
     assert np.median(np.abs(got['offsets'] - ref['offsets'])[same]) < 1e-5
 
 
-@pytest.mark.parametrize('variant', ['simple', 'padded', 'piped', 'wide'])
+@pytest.mark.parametrize('variant', ['simple', 'padded', 'piped', 'wide', 'quad'])
 def test_edge_peaks_w64_r48(gpu, variant):
     """Window 64, search 48 (M = 97: the edge-row and edge-column waves of the 'simple' kernel hold
     surface row and column 96): gross offsets put the true integer peak of each window on u or v in
@@ -240,9 +240,10 @@ def test_grouping_and_device_calls_agree(gpu, fault, method):
 @pytest.mark.parametrize('workload', ['small', 'fault'])
 def test_int8_kernel_variants_agree(gpu, small, fault, workload):
     """The one-frame-at-a-time kernel with the edge row and column waves (search 48 and 16: M = 16 c + 1),
-    the same with full 16-row bands, the pipelined producer/helper kernel and the 32 x 32 kernel (M =
-    32 c + 1): the same offsets, peaks and covariances, the SNR to 1e-12 (the pipelined kernel splits its
-    sums over the helper waves); the 32 x 32 kernel gives the same bits as the 16 x 16 one throughout"""
+    the same with full 16-row bands, the pipelined producer/helper kernel, the 32 x 32 kernel (M =
+    32 c + 1) and the four-wave kernel: the same offsets, peaks and covariances, the SNR to 1e-12 (the
+    pipelined kernel splits its sums over the helper waves); the 32 x 32 and four-wave kernels give the
+    same bits as the 16 x 16 one throughout"""
     if workload == 'small':
         d, down, across, _, gross = small
         h, w, W, R, frames, g = 256, 320, d['window'], d['search'], d['frames'], gross
@@ -250,14 +251,15 @@ def test_int8_kernel_variants_agree(gpu, small, fault, workload):
         d, down, across, _ = fault
         h, w, W, R, frames, g = 720, 1280, 64, 48, d['frames'], None
     out = {}
-    for variant in ('simple', 'piped', 'padded', 'wide'):
+    for variant in ('simple', 'piped', 'padded', 'wide', 'quad'):
         cor = framecor.Correlator(h, w, down, across, window=W, search=R, method='int8')
         cor.set_variant(variant)
         cor.set_reference(d['ref'])
         out[variant] = cor.correlate(frames, gross=g)
     a = out['simple']
-    for key in a:
-        assert np.array_equal(a[key], out['wide'][key], equal_nan=True), ('wide', key)
+    for other in ('wide', 'quad'):
+        for key in a:
+            assert np.array_equal(a[key], out[other][key], equal_nan=True), (other, key)
     ok = a['snr'] > 0
     for other in ('padded', 'piped'):   # the same surface, its SNR sums split over other waves
         b = out[other]
