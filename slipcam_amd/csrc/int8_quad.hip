@@ -3,17 +3,17 @@
 // The same exact integer contraction, screen, candidates, SNR box and FP64 fit as the one-frame kernel
 // of int8_corr.hip (so the outputs are the same bits), for surfaces of M = 16 c + 1 points (search
 // ranges R = 8 c) with c = 4 or 6 and templates of 32 or 64 columns. There a block has c band waves and
-// two light edge waves, so two of
-// the four SIMDs of a CU carry twice the MFMA work of the other two. Here a block has four waves, one
-// per SIMD, and the c x c core tiles plus the edge row and column are dealt so that every wave carries
-// the same MFMA work to within one tile:
+// two light edge waves, so two of the four SIMDs of a CU carry twice the MFMA work of the other two.
+// Here a block has four waves, one per SIMD, and the c x c core tiles plus the edge row and column are
+// dealt so that every wave carries the same MFMA work to within one tile:
 //   c = 4: wave w the band w; the edge row to wave 0, the edge column to wave 1
 //   c = 6: wave w the band w and tiles 3 (w & 1) .. + 2 of band 4 + (w >> 1); edges to waves 0 and 1
 // A wave's two band segments and the edge row share the template's B fragments, built once per row
-// group of four. Three blocks run per CU (43 KB of LDS each at window 64, search 48): a SIMD holds one
-// wave of each, and the blocks are independent, so the per-frame screen and peak fit of one overlap
-// the MFMAs of the others. The peak fit of a frame alternates between waves 2 and 3, which carry no
-// edge.
+// group of four; the edge column runs on independent MFMA chains with its extent known at compile time.
+// The LDS holds the chip's L rows only and template rows of the narrowest conflict-free stride (112
+// bytes at window 64), 37 KB at window 64, search 48: four blocks run per CU, a SIMD holds one wave of
+// each, and the blocks are independent, so the per-frame screen and peak fit of one overlap the MFMAs
+// of the others. The peak fit of a frame alternates between waves 2 and 3, which carry no edge.
 
 #include "common.h"
 #include "int8_device.h"
@@ -385,6 +385,17 @@ __global__ void __launch_bounds__(kWave * kQuadWaves) __attribute__((amdgpu_wave
     quadWindow<C, NJB, NI4>(a, k, fBegin, fEnd, lds);
 }
 
+// One work item of the persistent form, as a function of its own. Inlined into the item loop of the
+// persistent kernel at 128 registers, the code of c = 6 came out wrong (every window of a window-64,
+// search-48 batch differed from the one-clip kernel, with no fault), while the same source compiled
+// with this call boundary, or with 168 registers, gives the one-clip kernel's bits; the batching tests
+// cover both shapes of this kernel.
+template <int C, int NJB, int NI4>
+__device__ __attribute__((noinline)) void quadItem(const Int8Args &a, int k, int fBegin, int fEnd, unsigned char *lds)
+{
+    quadWindow<C, NJB, NI4>(a, k, fBegin, fEnd, lds);
+}
+
 // the persistent form over the work queue of a tile shape (clips[c]: clip c's arguments)
 template <int C, int NJB, int NI4>
 __global__ void __launch_bounds__(kWave * kQuadWaves) __attribute__((amdgpu_waves_per_eu(4)))
@@ -399,7 +410,7 @@ int8QuadQueue(WorkQueue q, const Int8Args *__restrict__ clips)
         if (it < 0) break;
         int c, k, fBegin, fEnd;
         q.locateRunMajor(it, c, k, fBegin, fEnd);
-        quadWindow<C, NJB, NI4>(clips[c], k, fBegin, fEnd, lds);
+        quadItem<C, NJB, NI4>(clips[c], k, fBegin, fEnd, lds);
     }
 }
 

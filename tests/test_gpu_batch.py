@@ -109,6 +109,30 @@ def test_tuned_batch_replays_the_same(gpu):
         same(again[c], {k: v.cpu().numpy() for k, v in first[c].items()}, f'replay clip {c}')
 
 
+@pytest.mark.parametrize('variant', ['quad', 'simple'])
+def test_int8_batch_plans_search_32_and_48(gpu, variant):
+    """The four-wave kernel's shapes (windows 32 and 64, search 32 and 48: c = 4 and 6 core bands) under
+    explicit plans, 1 and 16 frames per item, 4 blocks per CU: the same bits as each clip alone (the
+    persistent form of c = 6 is compiled with each work item behind a call, which gives these bits)"""
+    specs = []
+    for i, (W, R) in enumerate(((32, 32), (32, 48), (64, 32), (64, 48))):
+        specs.append(synthetic.clip_specs(n=1, seed=140 + i, frames=(5, 5), resolutions=((480, 640),), windows=(W,),
+                                          searches=(R,))[0])
+    vids = videos(specs)
+    for fpb in (1, 16):
+        batch = framecor.Batch(specs, method='int8', tune=False, plan_cache=None)
+        for c, (ref, _) in enumerate(vids):
+            batch.set_reference(c, ref)
+            batch.set_plan(c, variant, fpb, 4)
+        out = batch.correlate([fr for _, fr in vids])
+        torch.cuda.synchronize()
+        for c, (s, (ref, fr)) in enumerate(zip(specs, vids)):
+            assert batch.plan(c)['variant'] == variant
+            expect = solo(s, ref, fr, 'int8', variant)
+            same(out[c], expect, f'clip {c} W {s["window"]} R {s["search"]} frames per item {fpb}')
+            assert np.isfinite(expect['offsets']).mean() > 0.5
+
+
 @pytest.mark.parametrize('method', ['int8', 'fft'])
 def test_batch_with_gross_offsets(gpu, method):
     specs = synthetic.clip_specs(n=4, seed=71, frames=(2, 5), windows=(32, 64), searches=(16, 32))
