@@ -184,6 +184,9 @@ __device__ __attribute__((always_inline)) inline void quadWindow(const Int8Args 
     int8_t *chip = tmpl + sh.chip;
     int *boxbuf = reinterpret_cast<int *>(lds + sh.box);
     WaveBest *parts = reinterpret_cast<WaveBest *>(lds + sh.parts);
+    // the raw values of the edge row (v = 0 .. M - 1) and the edge column (u = 0 .. M - 2) of the frame,
+    // kept in LDS by the wave that computes them rather than in registers of every wave
+    int *eRow = reinterpret_cast<int *>(lds + ((sh.total + 15) & ~15)), *eCol = eRow + M;
 
     loadTemplate(lds, sh.box, tmpl, a.tmpl + size_t(k) * W * W, W, a.tstride, tid, nthreads);
     const float normk = a.wd.norm[k];
@@ -218,12 +221,28 @@ __device__ __attribute__((always_inline)) inline void quadWindow(const Int8Args 
         int lane = lane0;
         asm volatile("" : "+v"(lane));
         const int g4 = lane >> 4, m16 = lane & 15;
-        v4i acc1[C], acc2[NT2 > 0 ? NT2 : 1], er, ec = v4i{0, 0, 0, 0};
-        if (erow)
+        v4i acc1[C], acc2[NT2 > 0 ? NT2 : 1];
+        if (erow) {
+            v4i er;
             quadMfma<C, NT2, NJB, true>(acc1, acc2, er, chip, tmpl, b1, b2, t2, C, lane, a.ni4, a.tstride, a.cstride);
-        else
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int blk = 4 * g4 + r, v = 16 * blk + m16;
+                if (blk <= C && v < M) eRow[v] = er[r];
+            }
+        }
+        else {
+            v4i er;
             quadMfma<C, NT2, NJB, false>(acc1, acc2, er, chip, tmpl, b1, b2, t2, C, lane, a.ni4, a.tstride, a.cstride);
-        if (ecol) ec = edgeColQuad<NI4>(chip, tmpl, zero, C, lane, a.tstride, a.cstride);
+        }
+        if (ecol) {
+            const v4i ec = edgeColQuad<NI4>(chip, tmpl, zero, C, lane, a.tstride, a.cstride);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int blk = 4 * g4 + r;
+                if (blk < C) eCol[16 * blk + m16] = ec[r];
+            }
+        }
 
         // this lane's points: fn(raw, u, v)
         auto visit = [&](auto &&fn) {
@@ -245,14 +264,14 @@ __device__ __attribute__((always_inline)) inline void quadWindow(const Int8Args 
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int blk = 4 * g4 + r, v = 16 * blk + m16;
-                    if (blk <= C && v < M) fn(er[r], 16 * C, v);
+                    if (blk <= C && v < M) fn(eRow[v], 16 * C, v);
                 }
             }
             if (ecol) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int blk = 4 * g4 + r;
-                    if (blk < C) fn(ec[r], 16 * blk + m16, 16 * C);
+                    const int blk = 4 * g4 + r, u = 16 * blk + m16;
+                    if (blk < C) fn(eCol[u], u, 16 * C);
                 }
             }
         };
@@ -271,7 +290,8 @@ __device__ __attribute__((always_inline)) inline void quadWindow(const Int8Args 
                 const int blk = 4 * g4 + r, v = 16 * blk + m16;
                 if (blk <= C && v < M) {
                     const int2 p = scr[16 * C * g.bw + v];
-                    top2Add(t, screenNcc(er[r], p.x, __int_as_float(p.y), mom), 16 * C * M + v, er[r]);
+                    const int raw = eRow[v];
+                    top2Add(t, screenNcc(raw, p.x, __int_as_float(p.y), mom), 16 * C * M + v, raw);
                 }
             }
         }
@@ -282,7 +302,8 @@ __device__ __attribute__((always_inline)) inline void quadWindow(const Int8Args 
                 const int blk = 4 * g4 + r, u = 16 * blk + m16;
                 if (blk < C) {
                     const int2 p = scr[u * g.bw + 16 * C];
-                    top2AddAny(t, screenNcc(ec[r], p.x, __int_as_float(p.y), mom), u * M + 16 * C, ec[r]);
+                    const int raw = eCol[u];
+                    top2AddAny(t, screenNcc(raw, p.x, __int_as_float(p.y), mom), u * M + 16 * C, raw);
                 }
             }
         }
@@ -344,14 +365,14 @@ __device__ __attribute__((always_inline)) inline void quadWindow(const Int8Args 
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int blk = 4 * g4 + r, v = 16 * blk + m16;
-                    if (blk <= C && v < M) put(er[r], 16 * C, v);
+                    if (blk <= C && v < M) put(eRow[v], 16 * C, v);
                 }
             }
             if (ecol && bx.vhi > 16 * C) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int blk = 4 * g4 + r;
-                    if (blk < C) put(ec[r], 16 * blk + m16, 16 * C);
+                    if (blk < C) put(eCol[16 * blk + m16], 16 * blk + m16, 16 * C);
                 }
             }
         }
@@ -427,7 +448,9 @@ int int8QuadThreads() { return kWave * kQuadWaves; }
 
 size_t int8QuadShmem(int ni4, int tstride, int crows, int cstride, int njb, const Geometry &g)
 {
-    return size_t(int8Lds(ni4, tstride, crows, cstride, njb, g.M, g.stat, kQuadWaves).total);
+    // the one-frame layout with four results, then the edge row and column values
+    const int total = int8Lds(ni4, tstride, crows, cstride, njb, g.M, g.stat, kQuadWaves).total;
+    return size_t(((total + 15) & ~15) + 8 * g.M);
 }
 
 // c core bands (M = 16 c + 1) and a template of W = 32 or 64 columns (njb blocks); null where none is
