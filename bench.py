@@ -40,10 +40,10 @@ FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector (packed), TFLOP/s
 # HBM traffic per launch of the two correlation kernels on this workload at 64 frames per launch, in KiB
 # per dispatch as rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE read them (one pass each), and the factors
 # that turn them into bytes: what the same counters read for a plain 256 MiB device copy
-# (profiles/counters_int8Correlate_fastCorrelate_720p_64frames.md holds the passes, the commands and the
-# commit they were measured on; measure again whenever a kernel changes)
-TRAFFIC_KIB = {'int8': (1.706e6, 4828.0), 'fft': (1.676e6, 84670.0)}
-TRAFFIC_SOURCE = 'profiles/counters_int8Correlate_fastCorrelate_720p_64frames.md (commit 9f12ae0)'
+# (profiles/counters_int8Quad_fastCorrelate_720p_64frames.md holds the passes, the commands and the commit
+# they were measured on; measure again whenever a kernel changes)
+TRAFFIC_KIB = {'int8': (3.769e6, 1.4423e6), 'fft': (1.676e6, 84663.0)}
+TRAFFIC_SOURCE = 'profiles/counters_int8Quad_fastCorrelate_720p_64frames.md (commit 8c5ff71)'
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 TRAFFIC_FRAMES = 64
 
@@ -429,7 +429,7 @@ def main():
                      'frac': round(achieved / INT8_PEAK_TOPS, 4),
                      'traffic': traffic_bytes('int8', i8['frames_per_launch']), 'traffic_source': TRAFFIC_SOURCE,
                      'algorithmic_bytes': i8_bytes_launch,
-                     'kernel': 'int8Correlate', 'kernel_ms_per_launch': round(i8['kernel_ms_per_launch'], 4),
+                     'kernel': 'int8Quad<6,5,16>', 'kernel_ms_per_launch': round(i8['kernel_ms_per_launch'], 4),
                      'ops_per_launch': ops_launch},
     }
     if not args.no_extras:
