@@ -406,11 +406,11 @@ __global__ void __launch_bounds__(kWave * kQuadWaves) __attribute__((amdgpu_wave
     quadWindow<C, NJB, NI4>(a, k, fBegin, fEnd, lds);
 }
 
-// One work item of the persistent form, as a function of its own. Inlined into the item loop of the
-// persistent kernel at 128 registers, the code of c = 6 came out wrong (every window of a window-64,
-// search-48 batch differed from the one-clip kernel, with no fault), while the same source compiled
-// with this call boundary, or with 168 registers, gives the one-clip kernel's bits; the batching tests
-// cover both shapes of this kernel.
+// One work item of the persistent form of c = 6, as a function of its own. Inlined into the item loop
+// of the persistent kernel at 128 registers, the code of c = 6 came out wrong (every window of a
+// window-64, search-48 batch differed from the one-clip kernel, with no fault), while the same source
+// compiled with this call boundary, or with 168 registers, gives the one-clip kernel's bits; c = 4 is
+// inlined. The batching tests cover both.
 template <int C, int NJB, int NI4>
 __device__ __attribute__((noinline)) void quadItem(const Int8Args &a, int k, int fBegin, int fEnd, unsigned char *lds)
 {
@@ -431,7 +431,10 @@ int8QuadQueue(WorkQueue q, const Int8Args *__restrict__ clips)
         if (it < 0) break;
         int c, k, fBegin, fEnd;
         q.locateRunMajor(it, c, k, fBegin, fEnd);
-        quadItem<C, NJB, NI4>(clips[c], k, fBegin, fEnd, lds);
+        if constexpr (C == 6)
+            quadItem<C, NJB, NI4>(clips[c], k, fBegin, fEnd, lds);
+        else
+            quadWindow<C, NJB, NI4>(clips[c], k, fBegin, fEnd, lds);
     }
 }
 
