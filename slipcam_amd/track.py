@@ -1,6 +1,6 @@
 #!/usr/bin/env python3
 """
-Dense offsets of the frames of a video relative to a reference frame, with framecor on an MI355X
+slipcam-track on an MI355X: per-window motion of every frame of a video against one reference frame
 
 The drop-in for the reference's slipcam-track (python/slipcam/track_offsets.py): each frame from
 --first to --last is correlated with the reference frame (--ref) on a grid of windows; the offsets are
@@ -103,19 +103,20 @@ def track(frames, ref, index, masked=None, window=64, skip=16, search=48, method
 
 def main(argv=None):
     parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
-    parser.add_argument('frames', help='the .npy of the frames (frames x height x width, uint8)')
-    parser.add_argument('--ref', type=int, required=True, help='the reference frame')
-    parser.add_argument('--first', type=int, default=0, help='the first frame (default: %(default)s)')
-    parser.add_argument('--last', type=int, help='the last frame (default: the last of the video)')
-    parser.add_argument('--window', type=int, default=64, help='the window size, pixels (default: %(default)s)')
-    parser.add_argument('--skip', type=int, default=16, help='the distance between windows, pixels (default: %(default)s)')
-    parser.add_argument('--search', type=int, default=48, help='the half search range, pixels (default: %(default)s)')
+    parser.add_argument('frames', help='video as a .npy array, n x height x width uint8 (memory-mapped)')
+    parser.add_argument('--ref', type=int, required=True, help='index of the frame every other frame is matched to')
+    parser.add_argument('--first', type=int, default=0, help='index of the first frame tracked (default: %(default)s)')
+    parser.add_argument('--last', type=int, help='index of the last frame tracked (default: the final frame)')
+    parser.add_argument('--window', type=int, default=64, help='side of each square window in pixels (default: %(default)s)')
+    parser.add_argument('--skip', type=int, default=16, help='grid spacing of the windows in pixels (default: %(default)s)')
+    parser.add_argument('--search', type=int, default=48,
+                        help='largest offset searched along each axis in pixels (default: %(default)s)')
     parser.add_argument('--mask', type=int, nargs=4, action='append', default=[], metavar=('X0', 'Y0', 'X1', 'Y1'),
-                        help='a box of burned-in text, pixels (repeatable)')
+                        help='pixel box of on-screen text whose windows are flagged; may be given several times')
     parser.add_argument('--method', choices=('fft', 'int8'), default='fft',
                         help='fft: the reference algorithm; int8: exact int8 matrix-core correlation (default: %(default)s)')
     parser.add_argument('--chunk', type=int, default=64, help='frames per correlation call (default: %(default)s)')
-    parser.add_argument('--out', required=True, help='the output .npz')
+    parser.add_argument('--out', required=True, help='path of the .npz written')
     args = parser.parse_args(argv)
 
     frames = np.load(args.frames, mmap_mode='r')
