@@ -23,9 +23,10 @@
 // One block per (window, run of frames); one wavefront per 16-row band of the surface (nut waves),
 // each over all v tiles (NVT) and all template rows. Per frame: the chip is staged in LDS as int8;
 // each wave accumulates its band in registers; the epilogue screens every point with an FP32 NCC,
-// takes the exact FP64 NCC of the points within FP32 error of the wave's maximum (so the argmax is
-// that of the exact FP64 surface, first maximum in row-major order on ties), and evaluates the SNR,
-// the 3 x 3 FP64 fit and the covariance on exact FP64 values (cxx/surface.h:71-146 formulas).
+// takes the exact FP64 NCC of the points within FP32 error of the block's maximum when there are
+// several (so the argmax is that of the exact FP64 surface, first maximum in row-major order on ties),
+// and evaluates the SNR, the 3 x 3 FP64 fit and the covariance on exact FP64 values
+// (cxx/surface.h:71-146 formulas).
 
 #include "common.h"
 #include "int8_device.h"
@@ -39,10 +40,10 @@ namespace {
 // row and one for the edge column.
 //
 // Per frame: every wave runs its MFMAs (a band, the edge row or the edge column) into registers and
-// screens its own points there (FP32 screen from the frame's screen map, exact FP64 NCC of the points
-// within the screen's error of the wave's maximum) into parts[wave]; barrier B2; every wave takes the
-// block's exact peak from parts and writes its raw values inside the SNR box around it to LDS, and the
-// chip of the next valid frame is staged (every MFMA of this frame is over); barrier B3; then the
+// screens its own points there (FP32 screen from the frame's screen map) into parts[wave]; barrier B2;
+// every wave takes the block's peak from parts (blockPeak: exact FP64 values only at near-ties) and
+// writes its raw values inside the SNR box around it to LDS, and the chip of the next valid frame is
+// staged (every MFMA of this frame is over); barrier B3; then the
 // finisher wave (the edge-row wave, whose MFMA share is the smallest, or wave 0) writes the outputs
 // while the other waves start the next frame's MFMAs. LDS: the template, a zero block, the chip, the
 // SNR box (min(stat, M)^2 int32) and the waves' results -- no full surface.
@@ -133,7 +134,7 @@ __device__ __attribute__((always_inline)) inline void int8Window(const Int8Args 
             }
         };
 
-        // the screen of this wave's points, then the exact best of its candidates
+        // the screen of this wave's points: its largest screened value and whether it stands alone
         const int64_t boff = f * g.boxSize + int64_t(fw.sy) * g.bw + fw.sx;
         const int2 *scr = a.fg.screen + boff;
         const int32_t *box2 = a.fg.box2 + boff;
@@ -151,47 +152,21 @@ __device__ __attribute__((always_inline)) inline void int8Window(const Int8Args 
 #pragma unroll
                 for (int v1 = 0; v1 < NVT; v1++)
                     if (EW != 0 || (u < M && 16 * v1 + m16 < M))
-                        top2Add(t, screenNcc(acc[v1][r], p[v1].x, __int_as_float(p[v1].y), mom), u * M + 16 * v1 + m16,
-                                acc[v1][r]);
+                        top2Add(t, screenNcc(acc[v1][r], p[v1].x, __int_as_float(p[v1].y), mom), u * M + 16 * v1 + m16);
             }
         }
         else
             visit([&](int raw, int u, int v) {
                 const int2 p = scr[u * g.bw + v];
-                top2Add(t, screenNcc(raw, p.x, __int_as_float(p.y), mom), u * M + v, raw);
+                top2Add(t, screenNcc(raw, p.x, __int_as_float(p.y), mom), u * M + v);
             });
-        float fmax = t.f1;
-        int imax = t.i1;
-        waveBest(fmax, imax);
-        const float thr = fmax - 1e-5f * fabsf(fmax) - 1e-30f;
-        double dbest = -3.0;
-        int dib = 0x7fffffff;
-        if (__any(t.f2 >= thr)) {
-            // a lane holds two candidates: every point within the screen's error of the maximum
-            visit([&](int raw, int u, int v) {
-                const int o = u * g.bw + v;
-                const int2 p = scr[o];
-                if (screenNcc(raw, p.x, __int_as_float(p.y), mom) >= thr)
-                    bestOf(dbest, dib, exactNcc(raw, p.x, box2[o], mom), u * M + v);
-            });
-        }
-        else if (t.f1 >= thr) {
-            const int u = t.i1 / M, v = t.i1 - u * M, o = u * g.bw + v;
-            dbest = exactNcc(t.r1, scr[o].x, box2[o], mom);
-            dib = t.i1;
-        }
-        waveBest(dbest, dib);
-        if (lane == 0) {
-            parts[wave].value = dbest;
-            parts[wave].index = dib;
-        }
+        screenPart(t, parts + wave, lane);
         __syncthreads();   // B2: every MFMA of this frame is over; parts are in
 
-        // the block's exact peak; this wave's raw values inside its SNR box
-        double best = -3.0;
-        int ib = 0x7fffffff;
-        for (int w = 0; w < nw; w++) bestOf(best, ib, parts[w].value, parts[w].index);
-        if (best > 0) {
+        // the block's peak (exact FP64 values only at near-ties); this wave's raw values inside its SNR box
+        int ib;
+        const bool valid = blockPeak(parts, nw, wave, lane, mom, scr, box2, g.bw, M, visit, ib);
+        if (valid) {
             const SnrBox b = snrBox(ib, M, g.stat);
             if (!band || (16 * wave < b.uhi && 16 * wave + 16 > b.ulo))
                 visit([&](int raw, int u, int v) {
@@ -211,7 +186,7 @@ __device__ __attribute__((always_inline)) inline void int8Window(const Int8Args 
             loadChip(chip, a.fg.frames + size_t(f) * g.pixels + size_t(fw.sy) * g.w + fw.sx, g.w, L, a.cstride, tid,
                      nthreads);
         __syncthreads();   // B3: the SNR box and the next chip are in
-        if (wave == finisher) finishPeak(a, mom, boxbuf, true, cur, fc, best, ib, lane);
+        if (wave == finisher) finishPeak(a, mom, boxbuf, true, cur, fc, valid, ib, lane);
     }
 }
 // The register budget of a block of NVT + EW waves: blocks of more than 8 waves run one per CU (no

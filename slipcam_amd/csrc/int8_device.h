@@ -54,25 +54,94 @@ __device__ inline float screenNcc(int raw, int fs, float rsig, const Moments &m)
     return float(numerator(raw, fs, m)) * rsig * m.rst;
 }
 
-// a lane's two largest screened values, the index (u M + v) and raw value of the largest; equal values:
-// the lower index
+// a lane's two largest screened values and the index (u M + v) of the largest; equal values: the lower
+// index
 struct Top2 {
     float f1, f2;
-    int i1, r1;
+    int i1;
 };
 
-__device__ inline Top2 top2Init() { return Top2{-3.0f, -3.0f, 0x7fffffff, 0}; }
+__device__ inline Top2 top2Init() { return Top2{-3.0f, -3.0f, 0x7fffffff}; }
 
 // A lane adds its points in ascending index order, so of equal values the first stays the largest and
 // only a strictly larger value replaces it. Branch-free: the second largest is the larger of the old
 // second and the smaller of (x, the old largest).
-__device__ inline void top2Add(Top2 &t, float x, int idx, int raw)
+__device__ inline void top2Add(Top2 &t, float x, int idx)
 {
     const bool win = x > t.f1;
     t.f2 = fmaxf(t.f2, fminf(x, t.f1));
     t.f1 = fmaxf(t.f1, x);
     t.i1 = win ? idx : t.i1;
-    t.r1 = win ? raw : t.r1;
+}
+
+// The screen's bound: the FP32 screen value of a point is within a few 1e-7 (relative) of its exact NCC,
+// so a point screened below screenThreshold(the largest screened value) is not the exact maximum.
+__device__ inline float screenThreshold(float fmax) { return fmax - 1e-5f * fabsf(fmax) - 1e-30f; }
+
+// A wave's screen result for its block, in *part: its largest screened value and that point's index, and
+// (pad = 1) whether no other point of the wave is screened within the bound of it.
+__device__ inline void screenPart(const Top2 &t, WaveBest *part, int lane)
+{
+    float fmax = t.f1;
+    int imax = t.i1;
+    waveBest(fmax, imax);
+    const float thr = screenThreshold(fmax);
+    const bool single = __popcll(__ballot(t.f1 >= thr)) == 1 && !__any(t.f2 >= thr);
+    if (lane == 0) {
+        part->value = fmax;
+        part->index = imax;
+        part->pad = single;
+    }
+}
+
+// The block's peak from its nw waves' screen parts (read behind the barrier that follows them). When one
+// point is screened above the bound of every other point of the block, it is the maximum of the exact
+// FP64 surface and no FP64 value is needed here. Otherwise (near-ties) every wave takes the exact NCC of
+// each of its points screened within the bound (visit(fn) calls fn(raw, u, v) for every point of the
+// wave) and the block the largest of them, the first in row-major order on equal values, behind two more
+// barriers: every wave decides from the same parts, so all of them take these or none. Returns whether
+// the peak is positive (else the frame's outputs are invalid); *ib is its index u M + v.
+template <typename Visit>
+__device__ inline bool blockPeak(WaveBest *parts, int nw, int wave, int lane, const Moments &mom, const int2 *scr,
+                                 const int32_t *box2, int bw, int M, Visit &&visit, int &ib)
+{
+    double top = -3.0;
+    int itop = 0x7fffffff, wtop = 0;
+    for (int w = 0; w < nw; w++) {
+        const double v = parts[w].value;
+        const int i = parts[w].index;
+        if (v > top || (v == top && i < itop)) {
+            top = v;
+            itop = i;
+            wtop = w;
+        }
+    }
+    const float thr = screenThreshold(float(top));
+    bool clear = parts[wtop].pad != 0;
+    for (int w = 0; w < nw; w++) clear = clear && (w == wtop || parts[w].value < thr);
+    if (clear) {
+        ib = itop;
+        return top > 0;
+    }
+    double dbest = -3.0;
+    int dib = 0x7fffffff;
+    visit([&](int raw, int u, int v) {
+        const int o = u * bw + v;
+        const int2 p = scr[o];
+        if (screenNcc(raw, p.x, __int_as_float(p.y), mom) >= thr)
+            bestOf(dbest, dib, exactNcc(raw, p.x, box2[o], mom), u * M + v);
+    });
+    waveBest(dbest, dib);
+    __syncthreads();   // every wave has read the screen parts
+    if (lane == 0) {
+        parts[wave].value = dbest;
+        parts[wave].index = dib;
+    }
+    __syncthreads();
+    double best = -3.0;
+    ib = 0x7fffffff;
+    for (int w = 0; w < nw; w++) bestOf(best, ib, parts[w].value, parts[w].index);
+    return best > 0;
 }
 
 // the raw int32 surface of one 16-row band (rows 16 band .. + 15, all NVT v tiles) of a staged chip
@@ -364,15 +433,15 @@ __device__ inline SnrBox snrBox(int ib, int M, int stat)
     return b;
 }
 
-// one wave: the outputs of frame f from its exact peak (best, ib): the SNR over the stat box and the 3 x 3
-// FP64 fit and covariance on exact values (cxx/surface.h:71-146 formulas). The raw surface S is the whole
-// M x M surface, or (boxed) only the SNR box, row-major from its corner.
+// one wave: the outputs of frame f from its peak ib (u M + v; valid: the peak is positive): the SNR over
+// the stat box and the 3 x 3 FP64 fit and covariance on exact values (cxx/surface.h:71-146 formulas). The
+// raw surface S is the whole M x M surface, or (boxed) only the SNR box, row-major from its corner.
 __device__ inline void finishPeak(const Int8Args &a, const Moments &mom, const int *S, bool boxed, const FrameWindow &fw,
-                                  int f, double best, int ib, int lane)
+                                  int f, bool valid, int ib, int lane)
 {
     const Geometry &g = a.g;
     const int M = g.M;
-    if (!fw.valid || !(best > 0)) {
+    if (!fw.valid || !valid) {
         if (lane == 0) invalidateOut(fw.q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
         return;
     }
@@ -382,7 +451,8 @@ __device__ inline void finishPeak(const Int8Args &a, const Moments &mom, const i
     const SnrBox b = snrBox(ib, M, g.stat);
     const int sStride = boxed ? b.bwid : M, su = boxed ? b.ulo : 0, sv = boxed ? b.vlo : 0;
     double sum = 0.0;
-    // the stat box, kBatch points per lane in flight; lanes 0..8 also take the 3 x 3 neighbours
+    // the stat box, kBatch points per lane in flight; lanes 0..8 also take the 3 x 3 neighbourhood (lane 4
+    // the peak itself, the only one when the peak is on the surface's border)
     double mine = 0.0;
     for (int it0 = lane; it0 < b.items; it0 += kBatch * kWave) {
         int raw[kBatch], fs[kBatch], f2v[kBatch];
@@ -405,14 +475,13 @@ __device__ inline void finishPeak(const Int8Args &a, const Moments &mom, const i
                 sum += d * d;
             }
     }
-    if (b.inside && lane < 9) {
+    if (lane < 9 && (b.inside || lane == 4)) {
         const int u = b.ub + lane / 3 - 1, v = b.vb + lane % 3 - 1, o = u * g.bw + v;
         mine = exactNcc(S[(u - su) * sStride + v - sv], scr[o].x, box2[o], mom);
     }
     sum = waveSumD(sum);
     double fsq[3][3];
     for (int i = 0; i < 9; i++) fsq[i / 3][i % 3] = __shfl(mine, i, kWave);
-    fsq[1][1] = best;
     if (lane == 0)
         writePeakF64(fsq, b.inside, b.ub, b.vb, g.R, fw.gy, fw.gx, sum, b.items, mom.nd, fw.q, a.fg.offset, a.fg.peak,
                      a.fg.snr, a.fg.cov);
@@ -425,7 +494,7 @@ __device__ inline void helperFinish(const Int8Args &a, const Moments &mom, const
     double best = -3.0;
     int ib = 0x7fffffff;
     for (int t = 0; t < nh; t++) bestOf(best, ib, part[t].value, part[t].index);
-    finishPeak(a, mom, S, false, fw, f, best, ib, lane);
+    finishPeak(a, mom, S, false, fw, f, best > 0, ib, lane);
 }
 
 } // namespace

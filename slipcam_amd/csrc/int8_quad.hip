@@ -111,8 +111,7 @@ __device__ inline void screenTiles(Top2 &t, const v4i (&acc)[NT], const int2 *sc
         for (int v1 = 0; v1 < NT; v1++) p[v1] = row[16 * v1];
 #pragma unroll
         for (int v1 = 0; v1 < NT; v1++)
-            top2Add(t, screenNcc(acc[v1][r], p[v1].x, __int_as_float(p[v1].y), mom), u * M + 16 * (t0 + v1) + m16,
-                    acc[v1][r]);
+            top2Add(t, screenNcc(acc[v1][r], p[v1].x, __int_as_float(p[v1].y), mom), u * M + 16 * (t0 + v1) + m16);
     }
 }
 
@@ -152,22 +151,20 @@ __device__ inline v4i edgeColQuad(const int8_t *chip, const int8_t *tmpl, const 
 
 // top2Add for points in any order: of equal values the lower index is the largest, and an equal value
 // of either index counts as a second candidate
-__device__ inline void top2AddAny(Top2 &t, float x, int idx, int raw)
+__device__ inline void top2AddAny(Top2 &t, float x, int idx)
 {
     const bool win = x > t.f1 || (x == t.f1 && idx < t.i1);
     t.f2 = win ? t.f1 : fmaxf(t.f2, x);
     t.f1 = win ? x : t.f1;
     t.i1 = win ? idx : t.i1;
-    t.r1 = win ? raw : t.r1;
 }
 
 // Window k against frames fBegin .. fEnd - 1, by a block of four waves (C core bands, M = 16 C + 1).
-// Per frame: every wave runs its MFMAs into registers and screens its own points there (FP32 screen,
-// exact FP64 NCC of the points within the screen's error of the wave's maximum) into parts[wave];
-// barrier B2; every wave takes the block's exact peak from parts and writes its raw values inside the
-// SNR box around it to LDS, and the chip of the next valid frame is staged; barrier B3; then the
-// finisher (wave 2 or 3 on alternate frames) writes the outputs while the other waves start the next
-// frame's MFMAs.
+// Per frame: every wave runs its MFMAs into registers and screens its own points there (FP32 screen)
+// into parts[wave]; barrier B2; every wave takes the block's peak from parts (blockPeak: exact FP64
+// values only at near-ties) and writes its raw values inside the SNR box around it to LDS, and the chip
+// of the next valid frame is staged; barrier B3; then the finisher (wave 2 or 3 on alternate frames)
+// writes the outputs while the other waves start the next frame's MFMAs.
 template <int C, int NJB, int NI4>
 __device__ __attribute__((always_inline)) inline void quadWindow(const Int8Args &a, int k, int fBegin, int fEnd,
                                                                  unsigned char *lds)
@@ -276,8 +273,8 @@ __device__ __attribute__((always_inline)) inline void quadWindow(const Int8Args 
             }
         };
 
-        // the screen of this wave's points (ascending index within each list), then the exact best of
-        // its candidates
+        // the screen of this wave's points (ascending index within each list): its largest screened value
+        // and whether it stands alone
         const int64_t boff = f * g.boxSize + int64_t(fw.sy) * g.bw + fw.sx;
         const int2 *scr = a.fg.screen + boff;
         const int32_t *box2 = a.fg.box2 + boff;
@@ -291,7 +288,7 @@ __device__ __attribute__((always_inline)) inline void quadWindow(const Int8Args 
                 if (blk <= C && v < M) {
                     const int2 p = scr[16 * C * g.bw + v];
                     const int raw = eRow[v];
-                    top2Add(t, screenNcc(raw, p.x, __int_as_float(p.y), mom), 16 * C * M + v, raw);
+                    top2Add(t, screenNcc(raw, p.x, __int_as_float(p.y), mom), 16 * C * M + v);
                 }
             }
         }
@@ -303,43 +300,17 @@ __device__ __attribute__((always_inline)) inline void quadWindow(const Int8Args 
                 if (blk < C) {
                     const int2 p = scr[u * g.bw + 16 * C];
                     const int raw = eCol[u];
-                    top2AddAny(t, screenNcc(raw, p.x, __int_as_float(p.y), mom), u * M + 16 * C, raw);
+                    top2AddAny(t, screenNcc(raw, p.x, __int_as_float(p.y), mom), u * M + 16 * C);
                 }
             }
         }
-        float fmax = t.f1;
-        int imax = t.i1;
-        waveBest(fmax, imax);
-        const float thr = fmax - 1e-5f * fabsf(fmax) - 1e-30f;
-        double dbest = -3.0;
-        int dib = 0x7fffffff;
-        if (__any(t.f2 >= thr)) {
-            // a lane holds two candidates: every point within the screen's error of the maximum
-            visit([&](int raw, int u, int v) {
-                const int o = u * g.bw + v;
-                const int2 p = scr[o];
-                if (screenNcc(raw, p.x, __int_as_float(p.y), mom) >= thr)
-                    bestOf(dbest, dib, exactNcc(raw, p.x, box2[o], mom), u * M + v);
-            });
-        }
-        else if (t.f1 >= thr) {
-            const int u = t.i1 / M, v = t.i1 - u * M, o = u * g.bw + v;
-            dbest = exactNcc(t.r1, scr[o].x, box2[o], mom);
-            dib = t.i1;
-        }
-        waveBest(dbest, dib);
-        if (lane == 0) {
-            parts[wave].value = dbest;
-            parts[wave].index = dib;
-        }
+        screenPart(t, parts + wave, lane);
         __syncthreads();   // B2: every MFMA of this frame is over; parts are in
 
-        // the block's exact peak; this wave's raw values inside its SNR box
-        double best = -3.0;
-        int ib = 0x7fffffff;
-#pragma unroll
-        for (int w = 0; w < kQuadWaves; w++) bestOf(best, ib, parts[w].value, parts[w].index);
-        if (best > 0) {
+        // the block's peak (exact FP64 values only at near-ties); this wave's raw values inside its SNR box
+        int ib;
+        const bool valid = blockPeak(parts, kQuadWaves, wave, lane, mom, scr, box2, g.bw, M, visit, ib);
+        if (valid) {
             const SnrBox bx = snrBox(ib, M, g.stat);
             auto put = [&](int raw, int u, int v) {
                 if (u >= bx.ulo && u < bx.uhi && v >= bx.vlo && v < bx.vhi) boxbuf[(u - bx.ulo) * bx.bwid + v - bx.vlo] = raw;
@@ -389,7 +360,7 @@ __device__ __attribute__((always_inline)) inline void quadWindow(const Int8Args 
             loadChip(chip, a.fg.frames + size_t(f) * g.pixels + size_t(fw.sy) * g.w + fw.sx, g.w, L, a.cstride, tid,
                      nthreads);
         __syncthreads();   // B3: the SNR box and the next chip are in
-        if (wave == finisher) finishPeak(a, mom, boxbuf, true, cur, fc, best, ib, lane);
+        if (wave == finisher) finishPeak(a, mom, boxbuf, true, cur, fc, valid, ib, lane);
         finisher ^= 1;     // 2, 3, 2, ...
     }
 }

@@ -158,7 +158,7 @@ __device__ __attribute__((always_inline)) inline void int8WideWindow(const Int8A
             }
         };
 
-        // the screen of this wave's points, then the exact best of its candidates
+        // the screen of this wave's points: its largest screened value and whether it stands alone
         const int64_t boff = f * g.boxSize + int64_t(fw.sy) * g.bw + fw.sx;
         const int2 *scr = a.fg.screen + boff;
         const int32_t *box2 = a.fg.box2 + boff;
@@ -174,47 +174,21 @@ __device__ __attribute__((always_inline)) inline void int8WideWindow(const Int8A
                 for (int v1 = 0; v1 < NB; v1++) p[v1] = row[32 * v1];
 #pragma unroll
                 for (int v1 = 0; v1 < NB; v1++)
-                    top2Add(t, screenNcc(acc[v1][q], p[v1].x, __int_as_float(p[v1].y), mom), u * M + 32 * v1 + n32,
-                            acc[v1][q]);
+                    top2Add(t, screenNcc(acc[v1][q], p[v1].x, __int_as_float(p[v1].y), mom), u * M + 32 * v1 + n32);
             }
         }
         else
             visit([&](int raw, int u, int v) {
                 const int2 p = scr[u * g.bw + v];
-                top2Add(t, screenNcc(raw, p.x, __int_as_float(p.y), mom), u * M + v, raw);
+                top2Add(t, screenNcc(raw, p.x, __int_as_float(p.y), mom), u * M + v);
             });
-        float fmax = t.f1;
-        int imax = t.i1;
-        waveBest(fmax, imax);
-        const float thr = fmax - 1e-5f * fabsf(fmax) - 1e-30f;
-        double dbest = -3.0;
-        int dib = 0x7fffffff;
-        if (__any(t.f2 >= thr)) {
-            // a lane holds two candidates: every point within the screen's error of the maximum
-            visit([&](int raw, int u, int v) {
-                const int o = u * g.bw + v;
-                const int2 p = scr[o];
-                if (screenNcc(raw, p.x, __int_as_float(p.y), mom) >= thr)
-                    bestOf(dbest, dib, exactNcc(raw, p.x, box2[o], mom), u * M + v);
-            });
-        }
-        else if (t.f1 >= thr) {
-            const int u = t.i1 / M, v = t.i1 - u * M, o = u * g.bw + v;
-            dbest = exactNcc(t.r1, scr[o].x, box2[o], mom);
-            dib = t.i1;
-        }
-        waveBest(dbest, dib);
-        if (lane == 0) {
-            parts[wave].value = dbest;
-            parts[wave].index = dib;
-        }
+        screenPart(t, parts + wave, lane);
         __syncthreads();   // B2: every MFMA of this frame is over; parts are in
 
-        // the block's exact peak; this wave's raw values inside its SNR box
-        double best = -3.0;
-        int ib = 0x7fffffff;
-        for (int w = 0; w < nw; w++) bestOf(best, ib, parts[w].value, parts[w].index);
-        if (best > 0) {
+        // the block's peak (exact FP64 values only at near-ties); this wave's raw values inside its SNR box
+        int ib;
+        const bool valid = blockPeak(parts, nw, wave, lane, mom, scr, box2, g.bw, M, visit, ib);
+        if (valid) {
             const SnrBox b = snrBox(ib, M, g.stat);
             if (!band || (32 * wave < b.uhi && 32 * wave + 32 > b.ulo))
                 visit([&](int raw, int u, int v) {
@@ -234,7 +208,7 @@ __device__ __attribute__((always_inline)) inline void int8WideWindow(const Int8A
             loadChip(chip, a.fg.frames + size_t(f) * g.pixels + size_t(fw.sy) * g.w + fw.sx, g.w, L, a.cstride, tid,
                      nthreads);
         __syncthreads();   // B3: the SNR box and the next chip are in
-        if (wave == finisher) finishPeak(a, mom, boxbuf, true, cur, fc, best, ib, lane);
+        if (wave == finisher) finishPeak(a, mom, boxbuf, true, cur, fc, valid, ib, lane);
     }
 }
 
