@@ -6,9 +6,16 @@
 // box side min(stat_size, M) (one kernel instantiation and LDS layout); the FFT form: the same
 // search-window size L -- share a persistent kernel that drains a work queue of (clip, window, run of
 // frames) items across all of them (WorkQueue in common.h: one range per XCD, then stealing), so small
-// clips fill the GPU together and large ones spread over every CU. A call is: zero the queue counters (a kernel, not a memset node),
-// then per stage of frames (each clip's frames in groups its box-sum buffers hold) the box-sum maps of
-// every clip and one queue kernel per tile shape. That sequence is captured once into a hipGraph and
+// clips fill the GPU together and large ones spread over every CU. A call is: zero the queue counters (a
+// kernel, not a memset node), then per stage of frames (each clip's frames in groups its box-sum buffers
+// hold) the box-sum maps and one queue kernel per tile shape. The int8 form runs this as one chain: the
+// maps of every clip in one launch, then the shapes' kernels one after another (each fills every
+// register slot of the CUs at its tuned blocks per CU; run side by side they slowed each other, 6 %).
+// The FFT form gives every tile shape a branch stream of its own (at most kBranches, the shapes dealt
+// largest first to the least loaded): per stage the maps of the shape's clips and its kernel. The
+// shapes share nothing but the read-only argument table (each clip has its own maps, counters and
+// outputs), and the FFT kernels' one block per CU leaves room beside it, so the branches run side by
+// side (12 % less time on the 16-clip workload). That sequence is captured once into a hipGraph and
 // replayed while the buffers stay the same; the clips' arguments sit in a device table written before
 // the capture.
 //
@@ -58,6 +65,8 @@ struct CallPointers {
     bool operator==(const CallPointers &o) const { return p == o.p; }
 };
 
+constexpr int kBranches = 4;   // the branch streams of a call (the hardware queues a process gets by default)
+
 } // namespace
 
 struct framecor_batch {
@@ -74,9 +83,18 @@ struct framecor_batch {
     hipEvent_t lastLaunch = nullptr;
     bool launched = false;
     DeviceBuffer dArgs, dCounters;
-    // the box-map jobs of each stage ([stage][clip], the first jobCount[stage] in use) and their blocks
+    // the box-map launches of each stage (boxGroups of them: the int8 form one for every clip, the FFT
+    // form one per tile shape), index stage * boxGroups + group: jobCount jobs from jobFirst in dJobs,
+    // jobBlocks blocks
     DeviceBuffer dJobs;
-    std::vector<int> jobCount, jobBlocks;
+    std::vector<int> jobFirst, jobCount, jobBlocks;
+    int boxGroups = 1;
+    // the FFT form: the branch streams of a call and the events that fork them from `stream` and join
+    // them to it; branchOf[shape] its branch
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t> joined;
+    hipEvent_t forked = nullptr;
+    std::vector<int> branchOf;
     // the pointers of the call whose arguments are in dArgs (the autotuner rewrites entries of stage 0)
     const uint8_t *const *argFr = nullptr;
     const int32_t *const *argGross = nullptr;
@@ -93,6 +111,9 @@ struct framecor_batch {
         if (launched) (void)hipEventSynchronize(lastLaunch);
         dropGraph();
         if (lastLaunch) (void)hipEventDestroy(lastLaunch);
+        if (forked) (void)hipEventDestroy(forked);
+        for (hipEvent_t e : joined) (void)hipEventDestroy(e);
+        for (hipStream_t b : branch) (void)hipStreamDestroy(b);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -278,37 +299,50 @@ struct framecor_batch {
             }
         dArgs.ensure(host.size());
         FC_HIP_CHECK(hipMemcpy(dArgs.ptr, host.data(), host.size(), hipMemcpyHostToDevice));
-        // the box maps of each stage: one job per clip with frames in it
-        std::vector<BoxJob> jobs(size_t(stages) * n);
-        jobCount.assign(stages, 0);
-        jobBlocks.assign(stages, 0);
+        // the box maps of each stage's launches: one job per clip of the launch with frames in the stage
+        const int ns = int(shapes.size());
+        boxGroups = isInt8() ? 1 : ns;
+        std::vector<int> all(n);
+        for (int c = 0; c < n; c++) all[c] = c;
+        std::vector<BoxJob> jobs;
+        jobFirst.assign(size_t(stages) * boxGroups, 0);
+        jobCount.assign(size_t(stages) * boxGroups, 0);
+        jobBlocks.assign(size_t(stages) * boxGroups, 0);
         for (int st = 0; st < stages; st++)
-            for (int c = 0; c < n; c++) {
-                const framecor_correlator &k = *clips[c];
-                int first, count;
-                stageFrames(c, st, first, count);
-                if (count <= 0) continue;
-                BoxJob &j = jobs[size_t(st) * n + jobCount[st]++];
-                j.frames = fr[c] + size_t(first) * k.g.pixels;
-                j.box = nullptr;
-                j.box2 = isInt8() ? k.dBox2.as<int32_t>() : nullptr;
-                j.screen = isInt8() ? k.dScreen.as<int2>() : nullptr;
-                j.sigma = isInt8() ? nullptr : k.dSigma.as<float>();
-                j.n = count;
-                j.h = k.g.h;
-                j.w = k.g.w;
-                j.W = k.g.W;
-                j.first = jobBlocks[st];
-                jobBlocks[st] += boxJobBlocks(count, k.g.h, k.g.w, k.g.W);
+            for (int gi = 0; gi < boxGroups; gi++) {
+                const int x = st * boxGroups + gi;
+                jobFirst[x] = int(jobs.size());
+                for (int c : isInt8() ? all : shapes[gi].clips) {
+                    const framecor_correlator &k = *clips[c];
+                    int first, count;
+                    stageFrames(c, st, first, count);
+                    if (count <= 0) continue;
+                    BoxJob j{};
+                    j.frames = fr[c] + size_t(first) * k.g.pixels;
+                    j.box = nullptr;
+                    j.box2 = isInt8() ? k.dBox2.as<int32_t>() : nullptr;
+                    j.screen = isInt8() ? k.dScreen.as<int2>() : nullptr;
+                    j.sigma = isInt8() ? nullptr : k.dSigma.as<float>();
+                    j.n = count;
+                    j.h = k.g.h;
+                    j.w = k.g.w;
+                    j.W = k.g.W;
+                    j.first = jobBlocks[x];
+                    jobBlocks[x] += boxJobBlocks(count, k.g.h, k.g.w, k.g.W);
+                    jobs.push_back(j);
+                    jobCount[x]++;
+                }
             }
-        dJobs.ensure(sizeof(BoxJob) * jobs.size());
-        FC_HIP_CHECK(hipMemcpy(dJobs.ptr, jobs.data(), sizeof(BoxJob) * jobs.size(), hipMemcpyHostToDevice));
+        dJobs.ensure(sizeof(BoxJob) * std::max<size_t>(jobs.size(), 1));
+        if (!jobs.empty())
+            FC_HIP_CHECK(hipMemcpy(dJobs.ptr, jobs.data(), sizeof(BoxJob) * jobs.size(), hipMemcpyHostToDevice));
     }
 
-    // the box maps of every clip's frames of stage st: one launch
-    void launchStageBoxes(int st, hipStream_t s) const
+    // the box maps of launch gi of stage st (the int8 form: every clip; the FFT form: tile shape gi)
+    void launchBoxGroup(int st, int gi, hipStream_t s) const
     {
-        launchBoxJobs(dJobs.as<BoxJob>() + size_t(st) * clips.size(), jobCount[st], jobBlocks[st], s);
+        const int x = st * boxGroups + gi;
+        launchBoxJobs(dJobs.as<BoxJob>() + jobFirst[x], jobCount[x], jobBlocks[x], s);
     }
 
     // the stage-0 arguments of the clips of shape sh for the int8 kernel variant under trial (the table
@@ -343,21 +377,78 @@ struct framecor_batch {
         for (Shape &sh : shapes) prepare(sh);
     }
 
-    // the launch sequence of one call (captured into the graph)
+    // the FFT form: the branch of each shape, largest first (transform work of a call) to the least
+    // loaded branch; the streams and events of the branches in use (created once)
+    void assignBranches()
+    {
+        if (isInt8()) return;
+        const int ns = int(shapes.size());
+        std::vector<double> work(ns, 0.0);
+        for (int si = 0; si < ns; si++)
+            for (int c : shapes[si].clips) {
+                const Geometry &g = clips[c]->g;
+                work[si] += double(g.nwin) * frames[c] * double(g.L) * g.L;
+            }
+        std::vector<int> order(ns);
+        for (int si = 0; si < ns; si++) order[si] = si;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+        const int nb = std::min(ns, kBranches);
+        std::vector<double> load(nb, 0.0);
+        branchOf.assign(ns, 0);
+        for (int si : order) {
+            const int b = int(std::min_element(load.begin(), load.end()) - load.begin());
+            branchOf[si] = b;
+            load[b] += work[si];
+        }
+        if (!forked) FC_HIP_CHECK(hipEventCreateWithFlags(&forked, hipEventDisableTiming));
+        while (int(branch.size()) < nb) {
+            hipStream_t b = nullptr;
+            FC_HIP_CHECK(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
+            branch.push_back(b);
+            hipEvent_t e = nullptr;
+            FC_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            joined.push_back(e);
+        }
+    }
+
+    // the launch sequence of one call (captured into the graph): the int8 form on s alone; the FFT form
+    // every shape's stages on its branch, forked from s after the counters are zeroed and joined to s at
+    // the end
     void enqueueCall(hipStream_t s) const
     {
         const int sets = counterSets;
         int *counters = dCounters.as<int>();
         hipLaunchKernelGGL(zeroInts, dim3(std::max(1, (kXcds * sets + 255) / 256)), dim3(256), 0, s, counters, kXcds * sets);
         FC_HIP_CHECK(hipGetLastError());
+        if (isInt8()) {
+            int set = 0;
+            for (int st = 0; st < stages; st++) {
+                launchBoxGroup(st, 0, s);
+                for (const Shape &sh : shapes)
+                    for (WorkQueue q : queues(sh, st)) {
+                        q.counters = counters + kXcds * set++;
+                        launchShape(sh, q, stageArgs(st), s);
+                    }
+            }
+            return;
+        }
+        const int ns = int(shapes.size()), nb = std::min(ns, kBranches);
+        FC_HIP_CHECK(hipEventRecord(forked, s));
+        for (int b = 0; b < nb; b++) FC_HIP_CHECK(hipStreamWaitEvent(branch[b], forked, 0));
         int set = 0;
-        for (int st = 0; st < stages; st++) {
-            launchStageBoxes(st, s);
-            for (const Shape &sh : shapes)
-                for (WorkQueue q : queues(sh, st)) {
+        for (int si = 0; si < ns; si++) {
+            const hipStream_t bs = branch[branchOf[si]];
+            for (int st = 0; st < stages; st++) {
+                launchBoxGroup(st, si, bs);
+                for (WorkQueue q : queues(shapes[si], st)) {
                     q.counters = counters + kXcds * set++;
-                    launchShape(sh, q, stageArgs(st), s);
+                    launchShape(shapes[si], q, stageArgs(st), bs);
                 }
+            }
+        }
+        for (int b = 0; b < nb; b++) {
+            FC_HIP_CHECK(hipEventRecord(joined[b], branch[b]));
+            FC_HIP_CHECK(hipStreamWaitEvent(s, joined[b], 0));
         }
     }
 
@@ -373,7 +464,7 @@ struct framecor_batch {
         counters = dTune.as<int>();
         try {
             // the box sums of stage 0 (the samples read them)
-            launchStageBoxes(0, stream);
+            for (int gi = 0; gi < boxGroups; gi++) launchBoxGroup(0, gi, stream);
             for (Shape &sh : shapes) {
                 if (sh.tuned) continue;
                 // a sample of about 64 items per CU, and at least as many frames as the largest frames per
@@ -480,6 +571,7 @@ struct framecor_batch {
                 writeArgs(fr, gross, offset, peak, snr, cov);
             }
             prepareCall();
+            assignBranches();
             FC_HIP_CHECK(hipStreamSynchronize(stream));
             FC_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
             try {

@@ -200,10 +200,11 @@ def test_batching_workload_16_clips(gpu, method):
     specs = synthetic.clip_specs()
     vids = videos(specs)
     batch, out = run_batch(specs, vids, method, tune=method == 'int8')
-    # the maps of all 16 clips fit on the device at once: one stage, one box-map launch, one queue launch
-    # per tile shape (and the counters' fill kernel)
+    # the maps of all 16 clips fit on the device at once: one stage; one queue launch per tile shape and
+    # the counters' fill kernel, and the box maps in one launch (int8) or one per tile shape (fft)
     info = batch.info()
-    assert info['stages'] == 1 and info['graph_nodes'] == 2 + info['queues'], info
+    boxes = 1 if method == 'int8' else info['shapes']
+    assert info['stages'] == 1 and info['graph_nodes'] == 1 + boxes + info['queues'], info
     rng = np.random.default_rng(17)
     for c, (s, (ref, fr)) in enumerate(zip(specs, vids)):
         where = f'clip {c}: {s["height"]}p W{s["window"]} R{s["search"]} x{s["frames"]}'
