@@ -12,7 +12,7 @@
 //               vertical sums and prefix scans across columns (no table: the correlator never
 //               materialises the int64 tables; exact: a box of squares sums to at most
 //               W^2 * 65025 < 2^31 for W <= 181), and optionally a packed map of
-//               {sum F, 1 / sqrt(W^2 sum F^2 - (sum F)^2) as FP32 (0 for a flat box)} that every window
+//               {sum F, 1 / sqrt(W^2 sum F^2 - (sum F)^2) to FP32 accuracy (0 for a flat box)} that every window
 //               touching that position shares (one 8-byte load per surface point in the int8 form's
 //               screen), or the FFT form's map of sqrtf(float(W^2 sum F^2 - (sum F)^2) / float(W^2))
 //               (0 for a flat box): the FP32 deviation that cxx/surface.h:30-35 evaluates per surface
@@ -110,79 +110,137 @@ satCols(int h, int w, int64_t satSize, int64_t *sat, int64_t *sat2)
 
 // The W x W box sums straight from the frames (no table): one block of kBoxThreads threads holds
 // kBoxThreads consecutive input columns x0 .. (one per thread) and kBoxThreads - W + 1 output columns;
-// it walks kBoxRows output rows down the frame. Per row each thread moves the vertical W-row sums of
-// its column one row down (one pixel in, one out), the block scans them into prefix sums across the
-// columns, and each output is the difference of two prefixes W apart. The arithmetic is modulo 2^32
-// (uint32): the prefixes of squares may wrap, but every box sum is below 2^31, so the differences are
-// exact. Two LDS buffers (by row parity) let one row's prefixes be read while the next row's are made.
-constexpr int kBoxThreads = 512, kBoxRows = 64;
+// it walks kBoxRows output rows down the frame, kBoxRound rows per round. Per row each thread moves the
+// vertical W-row sums of its column one row down (one pixel in, one out; the pixels of a round are
+// loaded a round ahead), every wave scans them into prefix sums across its 64 columns with DPP row
+// shifts and row broadcasts, the waves' totals go through LDS (one barrier per round), and each output
+// is the difference of two block prefixes W apart (a second barrier per round). The arithmetic is
+// modulo 2^32 (uint32): the prefixes of squares may wrap, but every box sum is below 2^31, so the
+// differences are exact. W^2 sum F^2 - (sum F)^2 is an integer below 2^53, exact in FP64.
+constexpr int kBoxThreads = 512, kBoxRows = 64, kBoxRound = 4;
+
+// inclusive prefix sum over the 64 lanes of a wave: shifts by 1, 2, 4, 8 within each row of 16 lanes
+// (lanes shifted in from outside the row add 0), then lane 15 of each row into the next row and lane
+// 31 into the upper two
+__device__ inline uint32_t waveScanDpp(uint32_t v)
+{
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x111, 0xf, 0xf, true));    // row_shr:1
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x112, 0xf, 0xf, true));    // row_shr:2
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x114, 0xf, 0xf, true));    // row_shr:4
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x118, 0xf, 0xf, true));    // row_shr:8
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x142, 0xa, 0xf, false));   // row_bcast:15
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x143, 0xc, 0xf, false));   // row_bcast:31
+    return v;
+}
 
 // block (bx, by) of frame f: output columns bx (kBoxThreads - W + 1) .., rows by kBoxRows ..
 __device__ __attribute__((always_inline)) inline void boxMapsBlock(const uint8_t *frames, int h, int w, int W,
                                                                    int64_t boxSize, int32_t *box, int32_t *box2,
                                                                    int2 *screen, float *sigma, int bx, int by, int f)
 {
-    __shared__ uint32_t pre[2][2][kBoxThreads];
-    __shared__ uint32_t tot[2][2][kBoxThreads / kWave];
+    constexpr int R = kBoxRound, NW = kBoxThreads / kWave;
+    __shared__ uint32_t pre[R][2][kBoxThreads];
+    __shared__ uint32_t tot[R][2][NW];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
     const int bh = h - W + 1, bw = w - W + 1, tx = kBoxThreads - W + 1;
     const int x0 = bx * tx, y0 = by * kBoxRows;
     const int x = x0 + tid, yEnd = min(y0 + kBoxRows, bh);
     const bool mine = x < w;
-    const uint8_t *F = frames + size_t(f) * h * w + x;
+    // every pixel load is unconditional, at a clamped address inside the frame (column w - 1, row h - 1)
+    // where its value is not used: no branch around the loads, so each wait counts exactly the loads
+    // still in flight and the next round's loads stay in flight through this round
+    const uint8_t *F = frames + size_t(f) * h * w + min(x, w - 1);
+    // the vertical sums of rows y0 .. y0 + W - 1, eight loads in flight
     uint32_t vs = 0, vs2 = 0;
-    if (mine)
-        for (int i = 0; i < W; i++) {
+    {
+        int i = 0;
+        for (; i + 8 <= W; i += 8) {
+            uint32_t p[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) p[t] = F[size_t(y0 + i + t) * w];
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                vs += p[t];
+                vs2 += p[t] * p[t];
+            }
+        }
+        for (; i < W; i++) {
             const uint32_t p = F[size_t(y0 + i) * w];
             vs += p;
             vs2 += p * p;
         }
-    const int64_t n = int64_t(W) * W;
-    // the pixels entering and leaving the vertical sums at the next row
-    uint32_t pin = 0, pout = 0;
-    if (mine && y0 + 1 < yEnd) {
-        pin = F[size_t(y0 + W) * w];
-        pout = F[size_t(y0) * w];
+        if (!mine) vs = vs2 = 0;
     }
-    for (int y = y0; y < yEnd; y++) {
-        uint32_t s = vs, s2 = vs2;
-        if (y + 1 < yEnd) {
-            vs += pin - pout;
-            vs2 += pin * pin - pout * pout;
-            if (mine && y + 2 < yEnd) {
-                pin = F[size_t(y + 1 + W) * w];
-                pout = F[size_t(y + 1) * w];
+    const double nd = double(W) * W;
+    // the pixels entering (row y + W) and leaving (row y) the vertical sums after output row y, for the
+    // rows y = yr .. yr + R - 1 of a round (used only where a later row is output: then y + W < h)
+    auto loadRound = [&](int yr, uint32_t (&in)[R], uint32_t (&out)[R]) {
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int y = yr + k;
+            in[k] = F[size_t(min(y + W, h - 1)) * w];
+            out[k] = F[size_t(min(y, h - 1)) * w];
+        }
+    };
+    uint32_t cin[R], cout[R];
+    loadRound(y0, cin, cout);
+    for (int yr = y0; yr < yEnd; yr += R) {
+        uint32_t nin[R], nout[R];
+        loadRound(yr + R, nin, nout);
+        uint32_t s[R], s2[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            s[k] = vs;
+            s2[k] = vs2;
+            if (mine && yr + k + 1 < yEnd) {
+                vs += cin[k] - cout[k];
+                vs2 += cin[k] * cin[k] - cout[k] * cout[k];
             }
         }
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint32_t t = __shfl_up(s, o, kWave), t2 = __shfl_up(s2, o, kWave);
-            if (lane >= o) {
-                s += t;
-                s2 += t2;
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            s[k] = waveScanDpp(s[k]);
+            s2[k] = waveScanDpp(s2[k]);
+        }
+        if (lane == kWave - 1)
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                tot[k][0][wave] = s[k];
+                tot[k][1][wave] = s2[k];
             }
-        }
-        const int b = y & 1;
-        if (lane == kWave - 1) {
-            tot[b][0][wave] = s;
-            tot[b][1][wave] = s2;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+#pragma unroll
+            for (int j = 0; j < NW - 1; j++)
+                if (j < wave) {
+                    s[k] += tot[k][0][j];
+                    s2[k] += tot[k][1][j];
+                }
+            pre[k][0][tid] = s[k];
+            pre[k][1][tid] = s2[k];
         }
         __syncthreads();
-        for (int j = 0; j < wave; j++) {
-            s += tot[b][0][j];
-            s2 += tot[b][1][j];
-        }
-        pre[b][0][tid] = s;
-        pre[b][1][tid] = s2;
-        __syncthreads();
-        if (tid < tx && x < bw) {
-            const uint32_t lo = tid ? pre[b][0][tid - 1] : 0u, lo2 = tid ? pre[b][1][tid - 1] : 0u;
-            const int32_t bs = int32_t(pre[b][0][tid + W - 1] - lo), bs2 = int32_t(pre[b][1][tid + W - 1] - lo2);
-            const int64_t o = f * boxSize + int64_t(y) * bw + x;
-            if (box) box[o] = bs;
-            if (box2) box2[o] = bs2;
-            const int64_t v = n * bs2 - int64_t(bs) * bs;
-            if (screen) screen[o] = make_int2(bs, __float_as_int(v > 0 ? float(1.0 / sqrt(double(v))) : 0.0f));
-            if (sigma) sigma[o] = v > 0 ? sqrtf(float(v) / float(n)) : 0.0f;
+        if (tid < tx && x < bw)
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                const int y = yr + k;
+                if (y < yEnd) {
+                    const uint32_t lo = tid ? pre[k][0][tid - 1] : 0u, lo2 = tid ? pre[k][1][tid - 1] : 0u;
+                    const int32_t bs = int32_t(pre[k][0][tid + W - 1] - lo), bs2 = int32_t(pre[k][1][tid + W - 1] - lo2);
+                    const int64_t o = f * boxSize + int64_t(y) * bw + x;
+                    if (box) box[o] = bs;
+                    if (box2) box2[o] = bs2;
+                    // W^2 sum F^2 - (sum F)^2, exact: every term an integer below 2^53
+                    const double v = fma(nd, double(bs2), -(double(bs) * double(bs)));
+                    if (screen) screen[o] = make_int2(bs, __float_as_int(v > 0 ? rsqrtf(float(v)) : 0.0f));
+                    if (sigma) sigma[o] = v > 0 ? sqrtf(float(v) / float(nd)) : 0.0f;
+                }
+            }
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            cin[k] = nin[k];
+            cout[k] = nout[k];
         }
     }
 }
