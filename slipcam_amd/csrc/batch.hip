@@ -489,7 +489,7 @@ struct framecor_batch {
                 int lastTrial = -1;
                 for (int v : variants)
                     for (int fpb : {1, 2, 4, 8, 16})
-                        for (int perCu : {1, 2, 4}) {
+                        for (int perCu : {1, 2, 3, 4}) {
                             Shape cand = sh;
                             cand.variant = v;
                             if (cand.variant != lastTrial) {

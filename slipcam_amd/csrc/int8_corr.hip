@@ -598,6 +598,18 @@ Int8Layout int8Layout(const Geometry &g)
     l.alias = false;
     l.supported = l.njb > 0 && l.nvt <= 9 && l.shmem <= 160 * 1024;
     l.edge = g.M % 16 == 1 && g.M > 1;
+    // the edge form reads no chip row below the L of the search window (the padded bands do), and for
+    // templates of 113-128 columns it takes template rows of 176 bytes, the narrowest multiple of 16 that
+    // holds a row (the four rows of a fragment build then share one bank of their five): 52 KB at
+    // window 128, search 16, three blocks per CU against two
+    l.edgeTstride = l.tstride;
+    l.edgeCrows = l.crows;
+    l.edgeShmem = l.shmem;
+    if (l.edge && l.njb >= 8) {
+        l.edgeTstride = 16 * ((16 * l.njb + 20 + 15) / 16);
+        l.edgeCrows = g.L;
+        l.edgeShmem = size_t(int8Lds(l.ni4, l.edgeTstride, l.edgeCrows, l.cstride, l.njb, g.M, g.stat, l.nut + 1).total);
+    }
     // the pipelined kernel: two chips, two raw surfaces, two sets of helper results
     const int nh = kPipedWaves - l.nut;
     l.pipedShmem = size_t(4 * l.ni4) * l.tstride + 2 * size_t(l.crows) * l.cstride
@@ -682,7 +694,7 @@ Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue)
     }
     if (!k.func) throw std::invalid_argument("framecor: no int8 kernel for this window and search range");
     k.threads = kWave * (piped ? kPipedWaves : nvt + (edge ? 2 : 0));
-    k.shmem = piped ? l.pipedShmem : l.shmem;
+    k.shmem = piped ? l.pipedShmem : edge ? l.edgeShmem : l.shmem;
     k.variant = piped ? Int8Variant::Piped : edge ? Int8Variant::Simple : Int8Variant::Padded;
     allowLargeLds(k.func);
     return k;
@@ -691,11 +703,12 @@ Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue)
 void fillInt8Args(Int8Args &a, const Int8Layout &l, Int8Variant variant)
 {
     const bool wide = variant == Int8Variant::Wide && l.wideOk, quad = variant == Int8Variant::Quad && l.quadOk;
+    const bool edge = variant == Int8Variant::Simple && l.edge;
     a.nut = wide ? l.nb : l.nut;
     a.ni4 = l.ni4;
-    a.tstride = wide ? l.wideTstride : quad ? l.quadTstride : l.tstride;
+    a.tstride = wide ? l.wideTstride : quad ? l.quadTstride : edge ? l.edgeTstride : l.tstride;
     a.cstride = wide ? l.wideCstride : l.cstride;
-    a.crows = wide ? l.wideCrows : quad ? l.quadCrows : l.crows;
+    a.crows = wide ? l.wideCrows : quad ? l.quadCrows : edge ? l.edgeCrows : l.crows;
     a.alias = l.alias ? 1 : 0;
 }
 

@@ -71,6 +71,8 @@ struct Int8Layout {
     bool alias;           // (unused: the one-frame kernel keeps no full surface)
     bool supported;
     bool edge;            // M = 16 c + 1: c core bands plus a wave for the edge row and one for the edge column
+    int edgeTstride, edgeCrows;   // the one-frame kernel with the edge waves (variant simple): its LDS
+    size_t edgeShmem;
     size_t pipedShmem;    // the pipelined producer/helper kernel
     bool pipedOk;
     // the 32 x 32 kernel (int8_wide.hip) where M = 32 nb + 1: nb bands, nj template blocks, its LDS
