@@ -140,19 +140,30 @@ __device__ __attribute__((always_inline)) inline void int8Window(const Int8Args 
         const int32_t *box2 = a.fg.box2 + boff;
         Top2 t = top2Init();
         if (band) {
-            // a row of NVT points per r: one row pointer, the columns at immediate offsets
+            // a row of NVT points per r: one row pointer, the columns at immediate offsets; up to three
+            // tiles a band (search 16), the loads of all four rows issued first (at four tiles, the
+            // registers cost more than the waits save: 3-4 % slower at window 32 / search 32)
+            constexpr int RL = NVT <= 3 ? 4 : 1;   // rows whose loads are in flight together
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int u = 16 * wave + 4 * g4 + r;
-                const int2 *row = scr + u * g.bw + m16;
-                int2 p[NVT];
+            for (int r0 = 0; r0 < 4; r0 += RL) {
+                int2 p[RL][NVT];
 #pragma unroll
-                for (int v1 = 0; v1 < NVT; v1++)
-                    p[v1] = (EW != 0 || (u < M && 16 * v1 + m16 < M)) ? row[16 * v1] : make_int2(0, 0);
+                for (int r = 0; r < RL; r++) {
+                    const int u = 16 * wave + 4 * g4 + r0 + r;
+                    const int2 *row = scr + u * g.bw + m16;
 #pragma unroll
-                for (int v1 = 0; v1 < NVT; v1++)
-                    if (EW != 0 || (u < M && 16 * v1 + m16 < M))
-                        top2Add(t, screenNcc(acc[v1][r], p[v1].x, __int_as_float(p[v1].y), mom), u * M + 16 * v1 + m16);
+                    for (int v1 = 0; v1 < NVT; v1++)
+                        p[r][v1] = (EW != 0 || (u < M && 16 * v1 + m16 < M)) ? row[16 * v1] : make_int2(0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < RL; r++) {
+                    const int u = 16 * wave + 4 * g4 + r0 + r;
+#pragma unroll
+                    for (int v1 = 0; v1 < NVT; v1++)
+                        if (EW != 0 || (u < M && 16 * v1 + m16 < M))
+                            top2Add(t, screenNcc(acc[v1][r0 + r], p[r][v1].x, __int_as_float(p[r][v1].y), mom),
+                                    u * M + 16 * v1 + m16);
+                }
             }
         }
         else

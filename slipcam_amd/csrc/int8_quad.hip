@@ -97,21 +97,41 @@ __device__ inline void quadMfma(v4i (&acc1)[NT1], v4i (&acc2)[NT2 > 0 ? NT2 : 1]
 }
 
 // the FP32 screen of NT tiles of band `band` from tile t0: rows in ascending order, columns ascending
-// within a row (top2Add's order)
+// within a row (top2Add's order); up to four tiles with the loads of all four rows issued first
 template <int NT>
 __device__ inline void screenTiles(Top2 &t, const v4i (&acc)[NT], const int2 *scr, int bw, int M, int band, int t0,
                                    int g4, int m16, const Moments &mom)
 {
+    if constexpr (NT <= 4) {
+        // every row's map loads in flight together (at most 16 a lane), then the points in order
+        int2 p[4][NT];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int u = 16 * band + 4 * g4 + r;
-        const int2 *row = scr + u * bw + 16 * t0 + m16;
-        int2 p[NT];
+        for (int r = 0; r < 4; r++) {
+            const int2 *row = scr + (16 * band + 4 * g4 + r) * bw + 16 * t0 + m16;
 #pragma unroll
-        for (int v1 = 0; v1 < NT; v1++) p[v1] = row[16 * v1];
+            for (int v1 = 0; v1 < NT; v1++) p[r][v1] = row[16 * v1];
+        }
 #pragma unroll
-        for (int v1 = 0; v1 < NT; v1++)
-            top2Add(t, screenNcc(acc[v1][r], p[v1].x, __int_as_float(p[v1].y), mom), u * M + 16 * (t0 + v1) + m16);
+        for (int r = 0; r < 4; r++) {
+            const int u = 16 * band + 4 * g4 + r;
+#pragma unroll
+            for (int v1 = 0; v1 < NT; v1++)
+                top2Add(t, screenNcc(acc[v1][r], p[r][v1].x, __int_as_float(p[r][v1].y), mom),
+                        u * M + 16 * (t0 + v1) + m16);
+        }
+    }
+    else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int u = 16 * band + 4 * g4 + r;
+            const int2 *row = scr + u * bw + 16 * t0 + m16;
+            int2 p[NT];
+#pragma unroll
+            for (int v1 = 0; v1 < NT; v1++) p[v1] = row[16 * v1];
+#pragma unroll
+            for (int v1 = 0; v1 < NT; v1++)
+                top2Add(t, screenNcc(acc[v1][r], p[v1].x, __int_as_float(p[v1].y), mom), u * M + 16 * (t0 + v1) + m16);
+        }
     }
 }
 
