@@ -38,10 +38,12 @@ def workload(W, R, n=2, seed=0):
 
 # the 32 x 32 kernel (search 16, 32, 48, 64) at every template width class it instantiates:
 # (3, 16), (20, 16), (40, 48), (64, 16/32), (80, 32), (90, 16), (100, 48), (125, 64), (128, 64);
-# the four-wave kernel at each of its shapes: (32, 32), (32, 48), (64, 32), (64, 48)
+# the four-wave kernel at each of its shapes: (32, 32), (32, 48), (64, 32), (64, 48); the one-frame kernel
+# with the edge waves and 113-128 template columns (the chip's L rows, 176-byte template rows): (100, 48),
+# (125, 64), (128, 16), (128, 32), (128, 64)
 INT8_SHAPES = [(3, 1), (16, 8), (20, 5), (33, 17), (48, 24), (40, 56), (64, 16), (64, 32), (96, 40), (128, 64),
                (129, 71), (3, 16), (20, 16), (40, 48), (80, 32), (90, 16), (100, 48), (125, 64), (32, 32), (32, 48),
-               (64, 48)]
+               (64, 48), (128, 16), (128, 32)]
 
 
 @pytest.mark.parametrize('W,R', INT8_SHAPES)
