@@ -728,10 +728,10 @@ void launchInt8(Int8Args a, const Int8Layout &l, Int8Variant variant, hipStream_
     if (a.fg.n <= 0) return;
     const Int8Kernel k = int8Kernel(l, variant, false);
     fillInt8Args(a, l, k.variant);
-    // frames per block: at most 16 (at 720p, window 64, search 48, 64 frames in blocks of 16 run 3 %
-    // faster than in blocks of 64, and the batching autotuner settles on 16 for most tile shapes), and
-    // enough blocks to fill the GPU several times over
-    const int target = 2048, maxFpb = 16;
+    // frames per block: at most 8 (at 720p, window 64, search 48, 64 frames in blocks of 8 run 2 % faster
+    // than in blocks of 16 with the four-wave kernel, blocks of 4, 6, 11 or 13 slower, blocks of 64 slower
+    // still), and enough blocks to fill the GPU several times over
+    const int target = 2048, maxFpb = 8;
     int nfb = std::max((a.fg.n + maxFpb - 1) / maxFpb, (target + a.g.nwin - 1) / a.g.nwin);
     if (nfb > a.fg.n) nfb = a.fg.n;
     if (nfb < 1) nfb = 1;

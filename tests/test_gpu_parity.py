@@ -271,8 +271,8 @@ def test_int8_kernel_variants_agree(gpu, small, fault, workload):
 
 @pytest.mark.parametrize('n', [37, 48])
 def test_int8_runs_of_frames_agree(gpu, n):
-    """A call of more than 16 frames is dealt to the blocks in runs of at most 16 frames (37: runs of
-    13, 13 and 11; 48: three of 16), run of frames major: the same bits as frame by frame"""
+    """A call of more than 8 frames is dealt to the blocks in runs of at most 8 frames (37: runs of 8, 8,
+    8, 8 and 5; 48: six of 8), run of frames major: the same bits as frame by frame"""
     h, w, W, R = 480, 640, 32, 16
     base = synthetic.texture(h, w, sigma=1.5, seed=n)
     rng = np.random.default_rng(n)
