@@ -136,8 +136,9 @@ int framecor_ncc_refine(framecor_correlator *c, const float *frames, int n, cons
  * the edge row and the edge column in a wave each), 4 one frame at a time on 32 x 32 tiles
  * (v_mfma_i32_32x32x32_i8) where M = 32 c + 1, c <= 4 (search 16, 32, 48, 64; else as 1), 5 one frame
  * at a time in blocks of four waves with equal MFMA shares where M = 16 c + 1, c = 4 or 6 (search 32 or
- * 48) and the window is 32 or 64 (else as 1); the offsets, peaks and covariances are the same, the SNR
- * to rounding */
+ * 48) and the window is 32 or 64 (else as 1), 6 one wave per window and frame over ceil(M / 16) <= 3
+ * full bands where the search is at most 23 and the window at most 64 (else as 1); the offsets, peaks
+ * and covariances are the same, the SNR to rounding */
 int framecor_set_variant(framecor_correlator *c, int variant);
 
 /* per-stage GPU timing (no counterpart in the reference; for benchmarks): with enable 1, the
@@ -189,9 +190,9 @@ int framecor_batch_set_reference(framecor_batch *b, int clip, const uint8_t *fra
 int framecor_batch_set_tuning(framecor_batch *b, int enable);
 
 /* the plan of clip's tile shape: plan[0] the int8 kernel variant (as framecor_set_variant; resolved:
- * 1 edge waves, 2 pipelined, 3 full bands, 4 32 x 32 tiles, 5 four-wave blocks), plan[1] frames per
- * work item (0: automatic), plan[2]
- * resident blocks per CU (0: as many as fit), plan[3] 1 if tuned or set */
+ * 1 edge waves, 2 pipelined, 3 full bands, 4 32 x 32 tiles, 5 four-wave blocks, 6 one wave per window),
+ * plan[1] frames per work item (0: automatic), plan[2] resident blocks per CU (0: as many as fit),
+ * plan[3] 1 if tuned or set */
 int framecor_batch_plan(const framecor_batch *b, int clip, int *plan);
 
 /* sets the plan of clip's tile shape (plan[0..2] as above) */

@@ -480,6 +480,7 @@ struct framecor_batch {
                     if (sh.layout.edge) variants.push_back(int(Int8Variant::Simple));
                     variants.push_back(int(Int8Variant::Padded));
                     if (sh.layout.pipedOk) variants.push_back(int(Int8Variant::Piped));
+                    if (sh.layout.soloOk) variants.push_back(int(Int8Variant::Solo));
                 }
                 else {
                     variants.push_back(int(Int8Variant::Auto));
@@ -489,7 +490,7 @@ struct framecor_batch {
                 int lastTrial = -1;
                 for (int v : variants)
                     for (int fpb : {1, 2, 4, 8, 16})
-                        for (int perCu : {1, 2, 3, 4}) {
+                        for (int perCu : {1, 2, 3, 4, 6, 8, 12, 16}) {
                             Shape cand = sh;
                             cand.variant = v;
                             if (cand.variant != lastTrial) {
@@ -717,8 +718,8 @@ int framecor_batch_set_plan(framecor_batch *b, int clip, const int *plan)
 {
     if (!b || !plan) return fail(FRAMECOR_EINVAL, "framecor: null argument");
     if (clip < 0 || clip >= int(b->clips.size())) return fail(FRAMECOR_EINVAL, "framecor: no clip " + std::to_string(clip));
-    if (plan[0] < 0 || plan[0] > 5 || plan[1] < 0 || plan[2] < 0 || plan[2] > 64)
-        return fail(FRAMECOR_EINVAL, "framecor: a plan is (variant 0..5, frames per item >= 0, blocks per CU 0..64)");
+    if (plan[0] < 0 || plan[0] > 6 || plan[1] < 0 || plan[2] < 0 || plan[2] > 64)
+        return fail(FRAMECOR_EINVAL, "framecor: a plan is (variant 0..6, frames per item >= 0, blocks per CU 0..64)");
     return guarded([&] {
         if (b->launched) FC_HIP_CHECK(hipEventSynchronize(b->lastLaunch));
         Shape &sh = b->shapes[b->shapeOf[clip]];

@@ -138,7 +138,7 @@ int framecor_ncc_refine(framecor_correlator *c, const float *frames, int n, cons
 int framecor_set_variant(framecor_correlator *c, int variant)
 {
     if (!c) return fail(FRAMECOR_EINVAL, "framecor: null argument");
-    if (variant < 0 || variant > 5) return fail(FRAMECOR_EINVAL, "framecor: the kernel variant must be 0, 1, 2, 3, 4 or 5");
+    if (variant < 0 || variant > 6) return fail(FRAMECOR_EINVAL, "framecor: the kernel variant must be 0 to 6");
     c->variant = static_cast<Int8Variant>(variant);
     return FRAMECOR_OK;
 }

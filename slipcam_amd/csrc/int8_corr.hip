@@ -375,6 +375,228 @@ __global__ void __launch_bounds__(64 * kPipedWaves) int8PipedQueue(WorkQueue q, 
     }
 }
 
+// ---- one wave per window (small surfaces)
+//
+// A surface of at most three 16-row bands (M <= 48) against a template of at most 64 columns gives a
+// block little MFMA work per frame against its fixed costs (two barriers among its waves, the rounds of
+// map loads, the finisher's tail), with only a few windows per CU in flight. Here one wave does all of
+// a window's frame: its full bands (as the padded kernel, each template fragment built once for all of
+// them), the screen of every point,
+// the peak (blockPeak with one part), the SNR box in LDS and the fit (finishPeak), waiting for no other
+// wave; at 13 KB of LDS (window 32, search 16) twelve windows share a CU. The same screen, peak and
+// finishing as the other kernels: the same bits.
+
+// the chip as loadChip stages it, by the one wave: up to eight rows' loads a lane in flight before any
+// is waited for (a row past the chip's end loads its last row and is not stored)
+__device__ inline void soloChip(int8_t *chip, const uint8_t *src, int w, int L, int cstride, int lane)
+{
+    const int cpr = L >> 4;
+    if (cpr > 0) {
+        const int rowsPer = kWave / cpr, r0 = lane / cpr, cc = lane - r0 * cpr;
+        if (r0 < rowsPer)
+            for (int rb = r0; rb < L; rb += 8 * rowsPer) {
+                uint4 v[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const int r = min(rb + i * rowsPer, L - 1);
+                    __builtin_memcpy(&v[i], src + size_t(r) * w + 16 * cc, 16);
+                }
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const int r = rb + i * rowsPer;
+                    if (r < L) {
+                        v[i].x ^= 0x80808080u;
+                        v[i].y ^= 0x80808080u;
+                        v[i].z ^= 0x80808080u;
+                        v[i].w ^= 0x80808080u;
+                        *reinterpret_cast<uint4 *>(chip + r * cstride + 16 * cc) = v[i];
+                    }
+                }
+            }
+    }
+    const int tail = L & 15;
+    if (tail)
+        for (int c = lane; c < L * tail; c += kWave) {
+            const int r = c / tail, j = 16 * cpr + (c - r * tail);
+            chip[r * cstride + j] = int8_t(src[size_t(r) * w + j] ^ 0x80);
+        }
+}
+
+// every band of the surface into acc[band][v tile], as bandMfma band by band, but with the template's B
+// fragments of a row group built once for all the bands
+template <int NVT, int NJB>
+__device__ inline void soloBands(v4i (&acc)[NVT][NVT], const int8_t *chip, const int8_t *tmpl, int lane, int ni4,
+                                 int tstride, int cstride)
+{
+    const int g4 = lane >> 4, m16 = lane & 15;
+    const int bbyte = 16 - m16, bdw = bbyte >> 2, bsh = bbyte & 3;
+#pragma unroll
+    for (int b = 0; b < NVT; b++)
+#pragma unroll
+        for (int v1 = 0; v1 < NVT; v1++) acc[b][v1] = v4i{0, 0, 0, 0};
+    const int8_t *crow = chip + (g4 + m16) * cstride;
+    const uint32_t *trow = reinterpret_cast<const uint32_t *>(tmpl + g4 * tstride) + bdw;
+    for (int i4 = 0; i4 < ni4; i4++) {
+        v4i bf[NJB];
+#pragma unroll
+        for (int jb = 0; jb < NJB; jb++) {
+            const uint32_t *t = trow + 4 * jb;
+            const uint32_t w0 = t[0], w1 = t[1], w2 = t[2], w3 = t[3], w4 = t[4];
+            bf[jb] = v4i{int(__builtin_amdgcn_alignbyte(w1, w0, bsh)), int(__builtin_amdgcn_alignbyte(w2, w1, bsh)),
+                         int(__builtin_amdgcn_alignbyte(w3, w2, bsh)), int(__builtin_amdgcn_alignbyte(w4, w3, bsh))};
+        }
+#pragma unroll
+        for (int b = 0; b < NVT; b++)
+#pragma unroll
+            for (int cb = 0; cb < NVT + NJB - 1; cb++) {
+                const v4i av = *reinterpret_cast<const v4i *>(crow + 16 * b * cstride + 16 * cb);
+#pragma unroll
+                for (int jb = 0; jb < NJB; jb++) {
+                    const int v1 = cb - jb;
+                    if (v1 >= 0 && v1 < NVT) acc[b][v1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bf[jb], acc[b][v1], 0, 0, 0);
+                }
+            }
+        crow += 4 * cstride;
+        trow += tstride;
+    }
+}
+
+template <int NVT, int NJB>
+__device__ __attribute__((always_inline)) inline void int8SoloWindow(const Int8Args &a, int k, int fBegin, int fEnd,
+                                                                     unsigned char *lds)
+{
+    const Geometry &g = a.g;
+    const int lane0 = threadIdx.x & (kWave - 1);
+    const int W = g.W, M = g.M, L = g.L;
+    const Int8Lds sh = int8Lds(a.ni4, a.tstride, a.crows, a.cstride, NJB, M, g.stat, 1);
+    int8_t *tmpl = reinterpret_cast<int8_t *>(lds);
+    int8_t *chip = tmpl + sh.chip;
+    int *boxbuf = reinterpret_cast<int *>(lds + sh.box);
+    WaveBest *parts = reinterpret_cast<WaveBest *>(lds + sh.parts);
+
+    loadTemplate(lds, sh.box, tmpl, a.tmpl + size_t(k) * W * W, W, a.tstride, lane0, kWave);
+    const float normk = a.wd.norm[k];
+    const int y0 = a.wd.down[k / g.nx], x0 = a.wd.across[k % g.nx];
+    Moments mom;
+    mom.n = int64_t(W) * W;
+    mom.tvar = a.wd.tvar[k];
+    mom.nd = double(mom.n);
+    mom.tau = double(a.wd.tsum[k] - 128 * mom.n);
+    mom.rst = mom.tvar > 0 ? float(1.0 / sqrt(double(mom.tvar))) : 0.0f;
+    __syncthreads();   // the template (and the zeroed areas)
+
+    for (int f = fBegin; f < fEnd; f++) {
+        const FrameWindow fw = frameWindow(a, k, f, y0, x0, normk);
+        if (!fw.valid) {
+            if (lane0 == 0) invalidateOut(fw.q, a.fg.offset, a.fg.peak, a.fg.snr, a.fg.cov);
+            continue;
+        }
+        soloChip(chip, a.fg.frames + size_t(f) * g.pixels + size_t(fw.sy) * g.w + fw.sx, g.w, L, a.cstride, lane0);
+        __syncthreads();   // the chip
+        int lane = lane0;
+        asm volatile("" : "+v"(lane));
+        const int g4 = lane >> 4, m16 = lane & 15;
+        v4i acc[NVT][NVT];
+        soloBands<NVT, NJB>(acc, chip, tmpl, lane, a.ni4, a.tstride, a.cstride);
+
+        // this wave's points: fn(raw, u, v), ascending index
+        auto visit = [&](auto &&fn) {
+#pragma unroll
+            for (int b = 0; b < NVT; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int u = 16 * b + 4 * g4 + r;
+#pragma unroll
+                    for (int v1 = 0; v1 < NVT; v1++) {
+                        const int v = 16 * v1 + m16;
+                        if (u < M && v < M) fn(acc[b][v1][r], u, v);
+                    }
+                }
+        };
+        const int64_t boff = f * g.boxSize + int64_t(fw.sy) * g.bw + fw.sx;
+        const int2 *scr = a.fg.screen + boff;
+        const int32_t *box2 = a.fg.box2 + boff;
+        // the screen, band by band, the map loads of a band's four rows in flight together
+        Top2 t = top2Init();
+#pragma unroll
+        for (int b = 0; b < NVT; b++) {
+            int2 p[4][NVT];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int u = 16 * b + 4 * g4 + r;
+                const int2 *row = scr + u * g.bw + m16;
+#pragma unroll
+                for (int v1 = 0; v1 < NVT; v1++)
+                    p[r][v1] = (u < M && 16 * v1 + m16 < M) ? row[16 * v1] : make_int2(0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int u = 16 * b + 4 * g4 + r;
+#pragma unroll
+                for (int v1 = 0; v1 < NVT; v1++)
+                    if (u < M && 16 * v1 + m16 < M)
+                        top2Add(t, screenNcc(acc[b][v1][r], p[r][v1].x, __int_as_float(p[r][v1].y), mom),
+                                u * M + 16 * v1 + m16);
+            }
+        }
+        screenPart(t, parts, lane);
+        __syncthreads();
+        int ib;
+        const bool valid = blockPeak(parts, 1, 0, lane, mom, scr, box2, g.bw, M, visit, ib);
+        if (valid) {
+            const SnrBox b = snrBox(ib, M, g.stat);
+            visit([&](int raw, int u, int v) {
+                if (u >= b.ulo && u < b.uhi && v >= b.vlo && v < b.vhi) boxbuf[(u - b.ulo) * b.bwid + v - b.vlo] = raw;
+            });
+        }
+        __syncthreads();   // the SNR box
+        finishPeak(a, mom, boxbuf, true, fw, f, valid, ib, lane);
+        __syncthreads();   // the box and the chip read: the next frame may rewrite them
+    }
+}
+
+// three bands: 13 KB of LDS or more a block leaves room for twelve per CU (three waves per SIMD), whose
+// registers (168) hold the nine accumulators and the screen without spills (at 128 they spill)
+constexpr int soloWavesPerEu(int nvt) { return nvt >= 3 ? 3 : 4; }
+
+template <int NVT, int NJB>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(soloWavesPerEu(NVT)))) int8Solo(Int8Args a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int logical = xcdRemap(blockIdx.x, gridDim.x);
+    const int fblock = logical / a.g.nwin, k = logical - fblock * a.g.nwin;
+    const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n);
+    int8SoloWindow<NVT, NJB>(a, k, fBegin, fEnd, lds);
+}
+
+template <int NVT, int NJB>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(soloWavesPerEu(NVT))))
+int8SoloQueue(WorkQueue q, const Int8Args *__restrict__ clips)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int item[2];
+    for (int round = 0;; round++) {
+        if (threadIdx.x == 0) item[round & 1] = q.next(blockIdx.x);
+        __syncthreads();   // also: the previous item is over (its LDS may be rewritten)
+        const int it = __builtin_amdgcn_readfirstlane(item[round & 1]);
+        if (it < 0) break;
+        int c, k, fBegin, fEnd;
+        q.locateRunMajor(it, c, k, fBegin, fEnd);
+        int8SoloWindow<NVT, NJB>(clips[c], k, fBegin, fEnd, lds);
+    }
+}
+
+template <int NJB>
+const void *soloOf(int nvt, bool queue)
+{
+    switch (nvt) {
+    case 1: return queue ? reinterpret_cast<const void *>(int8SoloQueue<1, NJB>) : reinterpret_cast<const void *>(int8Solo<1, NJB>);
+    case 2: return queue ? reinterpret_cast<const void *>(int8SoloQueue<2, NJB>) : reinterpret_cast<const void *>(int8Solo<2, NJB>);
+    case 3: return queue ? reinterpret_cast<const void *>(int8SoloQueue<3, NJB>) : reinterpret_cast<const void *>(int8Solo<3, NJB>);
+    default: return nullptr;
+    }
+}
+
 // the kernels of one tile shape: kind 0 the one-frame kernel with full bands, 1 the same with the edge
 // row and column waves, 2 pipelined; direct (one block per window and run of frames) or queue
 template <int NVT, int NJB>
@@ -668,6 +890,9 @@ Int8Layout int8Layout(const Geometry &g)
     l.quadCrows = g.L;
     l.quadShmem = int8QuadShmem(l.ni4, l.quadTstride, l.quadCrows, l.cstride, l.njb, g);
     l.quadOk = l.supported && l.edge && l.quadShmem <= 80 * 1024 && int8QuadKernel(l.nvt - 1, g.W, l.njb, false);
+    // one wave per window: the padded layout with one screen part
+    l.soloShmem = size_t(int8Lds(l.ni4, l.tstride, l.crows, l.cstride, l.njb, g.M, g.stat, 1).total);
+    l.soloOk = l.supported && l.nvt <= 3 && l.njb <= 5;
     return l;
 }
 
@@ -689,6 +914,20 @@ Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue)
         k.threads = int8QuadThreads();
         k.shmem = l.quadShmem;
         k.variant = Int8Variant::Quad;
+        allowLargeLds(k.func);
+        return k;
+    }
+    if (l.soloOk && variant == Int8Variant::Solo) {
+        Int8Kernel k{};
+        switch (l.njb) {
+        case 2: k.func = soloOf<2>(l.nvt, queue); break;
+        case 3: k.func = soloOf<3>(l.nvt, queue); break;
+        case 5: k.func = soloOf<5>(l.nvt, queue); break;
+        default: break;
+        }
+        k.threads = kWave;
+        k.shmem = l.soloShmem;
+        k.variant = Int8Variant::Solo;
         allowLargeLds(k.func);
         return k;
     }

@@ -201,9 +201,11 @@ class Correlator(_framecor.Correlator):
         """The int8 kernel: 'auto' (as 'quad' where it applies, else 'simple'), 'simple' (one frame at a
         time, 16 x 16 tiles), 'piped' (producer/helper pipeline), 'padded' (one frame at a time, full
         16-row bands even where only an edge row and column remain), 'wide' (one frame at a time, 32 x 32
-        tiles, search ranges 16, 32, 48, 64; else as 'simple') or 'quad' (one frame at a time in blocks of
+        tiles, search ranges 16, 32, 48, 64; else as 'simple'), 'quad' (one frame at a time in blocks of
         four waves with equal MFMA shares, search ranges 32 and 48 with windows of 32 or 64; else as
-        'simple'); every variant gives the same offsets, peaks and covariances, the SNR to rounding"""
+        'simple') or 'solo' (one wave per window and frame, search ranges up to 23 with windows up to 64;
+        else as 'simple'); every variant gives the same offsets, peaks and covariances, the SNR to
+        rounding"""
         _check(_lib.framecor_set_variant(self._c, _VARIANTS[variant]))
 
     def set_timing(self, enable=True):
@@ -340,7 +342,11 @@ class _Clip(ctypes.Structure):
                 ('search', ctypes.c_int), ('stat_size', ctypes.c_int), ('frames', ctypes.c_int)]
 
 
-_VARIANTS = {'auto': 0, 'simple': 1, 'piped': 2, 'padded': 3, 'wide': 4, 'quad': 5}
+_VARIANTS = {'auto': 0, 'simple': 1, 'piped': 2, 'padded': 3, 'wide': 4, 'quad': 5, 'solo': 6}
+
+# the tuner's set of candidate plans, in the plan cache's key: plans tuned among an earlier set (before
+# the one-wave kernel and up to 16 blocks per CU) are tuned again rather than reused
+_PLAN_CANDIDATES = 2
 
 
 class Batch:
@@ -398,7 +404,8 @@ class Batch:
     def _device_key(self):
         import torch
         props = torch.cuda.get_device_properties(self._device)
-        return f'{self._method}|{getattr(props, "gcnArchName", props.name)}|{props.multi_processor_count}'
+        return (f'{self._method}|{getattr(props, "gcnArchName", props.name)}|{props.multi_processor_count}'
+                f'|candidates {_PLAN_CANDIDATES}')
 
     def _read_plans(self):
         """The cached plans of this batch's tile shapes (an unreadable cache is ignored)"""

@@ -133,6 +133,30 @@ def test_int8_batch_plans_search_32_and_48(gpu, variant):
             assert np.isfinite(expect['offsets']).mean() > 0.5
 
 
+def test_int8_batch_one_wave_plans(gpu):
+    """The one-wave kernel's shapes (search up to 23, windows up to 64: one, two and three 16-row bands)
+    under explicit plans, 1 and 16 frames per item, 8 and 12 blocks per CU: the same bits as each clip
+    alone with the same kernel and as the edge-wave kernel"""
+    specs = []
+    for i, (W, R) in enumerate(((16, 4), (32, 12), (32, 16), (64, 16), (48, 23))):
+        specs.append(synthetic.clip_specs(n=1, seed=160 + i, frames=(5, 5), resolutions=((480, 640),), windows=(W,),
+                                          searches=(R,))[0])
+    vids = videos(specs)
+    for fpb, per_cu in ((1, 12), (16, 8)):
+        batch = framecor.Batch(specs, method='int8', tune=False, plan_cache=None)
+        for c, (ref, _) in enumerate(vids):
+            batch.set_reference(c, ref)
+            batch.set_plan(c, 'solo', fpb, per_cu)
+        out = batch.correlate([fr for _, fr in vids])
+        torch.cuda.synchronize()
+        for c, (s, (ref, fr)) in enumerate(zip(specs, vids)):
+            assert batch.plan(c)['variant'] == 'solo'
+            expect = solo(s, ref, fr, 'int8', 'solo')
+            same(out[c], expect, f'clip {c} W {s["window"]} R {s["search"]} frames per item {fpb}')
+            same(expect, solo(s, ref, fr, 'int8', 'simple'), f'clip {c} against the edge-wave kernel')
+            assert np.isfinite(expect['offsets']).mean() > 0.5
+
+
 @pytest.mark.parametrize('method', ['int8', 'fft'])
 def test_batch_with_gross_offsets(gpu, method):
     specs = synthetic.clip_specs(n=4, seed=71, frames=(2, 5), windows=(32, 64), searches=(16, 32))

@@ -241,9 +241,9 @@ def test_grouping_and_device_calls_agree(gpu, fault, method):
 def test_int8_kernel_variants_agree(gpu, small, fault, workload):
     """The one-frame-at-a-time kernel with the edge row and column waves (search 48 and 16: M = 16 c + 1),
     the same with full 16-row bands, the pipelined producer/helper kernel, the 32 x 32 kernel (M =
-    32 c + 1) and the four-wave kernel: the same offsets, peaks and covariances, the SNR to 1e-12 (the
-    pipelined kernel splits its sums over the helper waves); the 32 x 32 and four-wave kernels give the
-    same bits as the 16 x 16 one throughout"""
+    32 c + 1), the four-wave kernel and the one-wave kernel (search up to 23): the same offsets, peaks and
+    covariances, the SNR to 1e-12 (the pipelined kernel splits its sums over the helper waves); the
+    32 x 32, four-wave and one-wave kernels give the same bits as the 16 x 16 one throughout"""
     if workload == 'small':
         d, down, across, _, gross = small
         h, w, W, R, frames, g = 256, 320, d['window'], d['search'], d['frames'], gross
@@ -251,13 +251,13 @@ def test_int8_kernel_variants_agree(gpu, small, fault, workload):
         d, down, across, _ = fault
         h, w, W, R, frames, g = 720, 1280, 64, 48, d['frames'], None
     out = {}
-    for variant in ('simple', 'piped', 'padded', 'wide', 'quad'):
+    for variant in ('simple', 'piped', 'padded', 'wide', 'quad', 'solo'):
         cor = framecor.Correlator(h, w, down, across, window=W, search=R, method='int8')
         cor.set_variant(variant)
         cor.set_reference(d['ref'])
         out[variant] = cor.correlate(frames, gross=g)
     a = out['simple']
-    for other in ('wide', 'quad'):
+    for other in ('wide', 'quad', 'solo'):
         for key in a:
             assert np.array_equal(a[key], out[other][key], equal_nan=True), (other, key)
     ok = a['snr'] > 0

@@ -56,7 +56,7 @@ def test_int8_shapes_vs_fp64_oracle(gpu, W, R):
     assert valid.any()
     cor = framecor.Correlator(d['h'], d['w'], d['down'], d['across'], window=W, search=R, method='int8')
     cor.set_reference(d['ref'])
-    for variant in ('simple', 'padded', 'piped', 'wide', 'quad'):
+    for variant in ('simple', 'padded', 'piped', 'wide', 'quad', 'solo'):
         cor.set_variant(variant)
         got = cor.correlate(d['frames'], gross=d['gross'])
         assert_int8_parity(got, ref)
