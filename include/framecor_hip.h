@@ -136,9 +136,10 @@ int framecor_ncc_refine(framecor_correlator *c, const float *frames, int n, cons
  * the edge row and the edge column in a wave each), 4 one frame at a time on 32 x 32 tiles
  * (v_mfma_i32_32x32x32_i8) where M = 32 c + 1, c <= 4 (search 16, 32, 48, 64; else as 1), 5 one frame
  * at a time in blocks of four waves with equal MFMA shares where M = 16 c + 1, c = 4 or 6 (search 32 or
- * 48) and the window is 32 or 64 (else as 1), 6 one wave per window and frame over ceil(M / 16) <= 3
- * full bands where the search is at most 23 and the window at most 64 (else as 1); the offsets, peaks
- * and covariances are the same, the SNR to rounding */
+ * 48) and the window is 32 or 64 (else as 1), 6 one wave per window and frame where the window is at
+ * most 64 and M = 16 c + 1 <= 65 (search 8, 16, 24 or 32: c bands, the edge row and column) or M <= 48
+ * (search up to 23: full bands) (else as 1); the offsets, peaks and covariances are the same, the SNR
+ * to rounding */
 int framecor_set_variant(framecor_correlator *c, int variant);
 
 /* per-stage GPU timing (no counterpart in the reference; for benchmarks): with enable 1, the

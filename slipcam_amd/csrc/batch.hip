@@ -472,7 +472,7 @@ struct framecor_batch {
                 // noise picks among them); at most the stage's frames of each clip
                 int64_t perFrame = 0;
                 for (int c : sh.clips) perFrame += clips[c]->g.nwin;
-                const int cap = int(std::max<int64_t>(16, std::min<int64_t>(64, (int64_t(cus) * 64 + perFrame - 1) / perFrame)));
+                const int cap = int(std::max<int64_t>(32, std::min<int64_t>(64, (int64_t(cus) * 64 + perFrame - 1) / perFrame)));
                 std::vector<int> variants;
                 if (isInt8()) {
                     if (sh.layout.quadOk) variants.push_back(int(Int8Variant::Quad));
@@ -489,7 +489,7 @@ struct framecor_batch {
                 Shape keep = sh;
                 int lastTrial = -1;
                 for (int v : variants)
-                    for (int fpb : {1, 2, 4, 8, 16})
+                    for (int fpb : {1, 2, 4, 8, 16, 32})
                         for (int perCu : {1, 2, 3, 4, 6, 8, 12, 16}) {
                             Shape cand = sh;
                             cand.variant = v;

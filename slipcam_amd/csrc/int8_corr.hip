@@ -377,14 +377,16 @@ __global__ void __launch_bounds__(64 * kPipedWaves) int8PipedQueue(WorkQueue q, 
 
 // ---- one wave per window (small surfaces)
 //
-// A surface of at most three 16-row bands (M <= 48) against a template of at most 64 columns gives a
-// block little MFMA work per frame against its fixed costs (two barriers among its waves, the rounds of
-// map loads, the finisher's tail), with only a few windows per CU in flight. Here one wave does all of
-// a window's frame: its full bands (as the padded kernel, each template fragment built once for all of
-// them), the screen of every point,
-// the peak (blockPeak with one part), the SNR box in LDS and the fit (finishPeak), waiting for no other
-// wave; at 13 KB of LDS (window 32, search 16) twelve windows share a CU. The same screen, peak and
-// finishing as the other kernels: the same bits.
+// A surface of at most four 16-row bands and an edge row and column (M = 16 c + 1 <= 65) or at most three
+// full bands (M <= 48), against a template of at most 64 columns, gives a block of several waves little
+// MFMA work per frame against its fixed costs (barriers among its waves, the rounds of map loads, the
+// finisher's tail), with only a few windows per CU in flight. Here one wave does all of a window's
+// frame: its bands (each template fragment built once for all of them), where M = 16 c + 1 the edge row
+// and column as the edge waves compute them, the screen of every point, the peak (blockPeak with one
+// part), the SNR box in LDS and the fit (finishPeak), waiting for no other wave. The template rows are
+// the four-wave kernel's (the narrowest conflict-free stride), the chip's rows its L rows (edge form) or
+// the full bands' reach: 10 KB a block at window 32, search 16 (sixteen windows per CU), 16 KB at
+// window 32, search 32 (ten). The same screen, peak and finishing as the other kernels: the same bits.
 
 // the chip as loadChip stages it, by the one wave: up to eight rows' loads a lane in flight before any
 // is waited for (a row past the chip's end loads its last row and is not stored)
@@ -461,7 +463,8 @@ __device__ inline void soloBands(v4i (&acc)[NVT][NVT], const int8_t *chip, const
     }
 }
 
-template <int NVT, int NJB>
+// NVT: the full bands (EW = 0) or the core bands c (EW = 2: with the edge row u = 16 c and column v = 16 c)
+template <int NVT, int NJB, int EW>
 __device__ __attribute__((always_inline)) inline void int8SoloWindow(const Int8Args &a, int k, int fBegin, int fEnd,
                                                                      unsigned char *lds)
 {
@@ -470,6 +473,7 @@ __device__ __attribute__((always_inline)) inline void int8SoloWindow(const Int8A
     const int W = g.W, M = g.M, L = g.L;
     const Int8Lds sh = int8Lds(a.ni4, a.tstride, a.crows, a.cstride, NJB, M, g.stat, 1);
     int8_t *tmpl = reinterpret_cast<int8_t *>(lds);
+    const int8_t *zero = tmpl + sh.zero;
     int8_t *chip = tmpl + sh.chip;
     int *boxbuf = reinterpret_cast<int *>(lds + sh.box);
     WaveBest *parts = reinterpret_cast<WaveBest *>(lds + sh.parts);
@@ -498,8 +502,28 @@ __device__ __attribute__((always_inline)) inline void int8SoloWindow(const Int8A
         const int g4 = lane >> 4, m16 = lane & 15;
         v4i acc[NVT][NVT];
         soloBands<NVT, NJB>(acc, chip, tmpl, lane, a.ni4, a.tstride, a.cstride);
+        v4i er = v4i{0, 0, 0, 0}, ec = v4i{0, 0, 0, 0};
+        if constexpr (EW != 0) {
+            er = edgeRowMfma<NJB>(chip, tmpl, NVT, lane, a.ni4, a.tstride, a.cstride);
+            ec = edgeColMfma<NJB>(chip, tmpl, zero, NVT, W, lane, a.tstride, a.cstride);
+        }
 
-        // this wave's points: fn(raw, u, v), ascending index
+        // this wave's points: fn(raw, u, v); the edge row's and column's (as their waves hold them) after
+        // the bands'
+        auto visitEdges = [&](auto &&fn) {
+            if constexpr (EW != 0) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int blk = 4 * g4 + r, v = 16 * blk + m16;
+                    if (blk <= NVT && v < M) fn(er[r], 16 * NVT, v);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int blk = 4 * g4 + r;
+                    if (blk < NVT) fn(ec[r], 16 * blk + m16, 16 * NVT);
+                }
+            }
+        };
         auto visit = [&](auto &&fn) {
 #pragma unroll
             for (int b = 0; b < NVT; b++)
@@ -509,9 +533,10 @@ __device__ __attribute__((always_inline)) inline void int8SoloWindow(const Int8A
 #pragma unroll
                     for (int v1 = 0; v1 < NVT; v1++) {
                         const int v = 16 * v1 + m16;
-                        if (u < M && v < M) fn(acc[b][v1][r], u, v);
+                        if (EW != 0 || (u < M && v < M)) fn(acc[b][v1][r], u, v);
                     }
                 }
+            visitEdges(fn);
         };
         const int64_t boff = f * g.boxSize + int64_t(fw.sy) * g.bw + fw.sx;
         const int2 *scr = a.fg.screen + boff;
@@ -527,18 +552,22 @@ __device__ __attribute__((always_inline)) inline void int8SoloWindow(const Int8A
                 const int2 *row = scr + u * g.bw + m16;
 #pragma unroll
                 for (int v1 = 0; v1 < NVT; v1++)
-                    p[r][v1] = (u < M && 16 * v1 + m16 < M) ? row[16 * v1] : make_int2(0, 0);
+                    p[r][v1] = (EW != 0 || (u < M && 16 * v1 + m16 < M)) ? row[16 * v1] : make_int2(0, 0);
             }
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int u = 16 * b + 4 * g4 + r;
 #pragma unroll
                 for (int v1 = 0; v1 < NVT; v1++)
-                    if (u < M && 16 * v1 + m16 < M)
+                    if (EW != 0 || (u < M && 16 * v1 + m16 < M))
                         top2Add(t, screenNcc(acc[b][v1][r], p[r][v1].x, __int_as_float(p[r][v1].y), mom),
                                 u * M + 16 * v1 + m16);
             }
         }
+        visitEdges([&](int raw, int u, int v) {
+            const int2 p = scr[u * g.bw + v];
+            top2Add(t, screenNcc(raw, p.x, __int_as_float(p.y), mom), u * M + v);
+        });
         screenPart(t, parts, lane);
         __syncthreads();
         int ib;
@@ -555,22 +584,23 @@ __device__ __attribute__((always_inline)) inline void int8SoloWindow(const Int8A
     }
 }
 
-// three bands: 13 KB of LDS or more a block leaves room for twelve per CU (three waves per SIMD), whose
-// registers (168) hold the nine accumulators and the screen without spills (at 128 they spill)
-constexpr int soloWavesPerEu(int nvt) { return nvt >= 3 ? 3 : 4; }
+// The register budget: three full bands, or the edge form's 2 x 2 or 3 x 3 core tiles, three waves per
+// SIMD (168 registers; at 128 they spill); the edge form's 4 x 4 core tiles two (256 registers), the
+// LDS of its blocks (16 KB at window 32) leaving room for ten per CU anyway
+constexpr int soloWavesPerEu(int nvt, int ew) { return ew != 0 && nvt >= 4 ? 2 : ew != 0 && nvt >= 2 ? 3 : nvt >= 3 ? 3 : 4; }
 
-template <int NVT, int NJB>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(soloWavesPerEu(NVT)))) int8Solo(Int8Args a)
+template <int NVT, int NJB, int EW>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(soloWavesPerEu(NVT, EW)))) int8Solo(Int8Args a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     const int logical = xcdRemap(blockIdx.x, gridDim.x);
     const int fblock = logical / a.g.nwin, k = logical - fblock * a.g.nwin;
     const int fBegin = fblock * a.fpb, fEnd = min(fBegin + a.fpb, a.fg.n);
-    int8SoloWindow<NVT, NJB>(a, k, fBegin, fEnd, lds);
+    int8SoloWindow<NVT, NJB, EW>(a, k, fBegin, fEnd, lds);
 }
 
-template <int NVT, int NJB>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(soloWavesPerEu(NVT))))
+template <int NVT, int NJB, int EW>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(soloWavesPerEu(NVT, EW))))
 int8SoloQueue(WorkQueue q, const Int8Args *__restrict__ clips)
 {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -582,17 +612,32 @@ int8SoloQueue(WorkQueue q, const Int8Args *__restrict__ clips)
         if (it < 0) break;
         int c, k, fBegin, fEnd;
         q.locateRunMajor(it, c, k, fBegin, fEnd);
-        int8SoloWindow<NVT, NJB>(clips[c], k, fBegin, fEnd, lds);
+        int8SoloWindow<NVT, NJB, EW>(clips[c], k, fBegin, fEnd, lds);
     }
 }
 
-template <int NJB>
-const void *soloOf(int nvt, bool queue)
+template <int NVT, int NJB, int EW>
+const void *soloKernel(bool queue)
 {
+    return queue ? reinterpret_cast<const void *>(int8SoloQueue<NVT, NJB, EW>)
+                 : reinterpret_cast<const void *>(int8Solo<NVT, NJB, EW>);
+}
+
+// nvt: the full bands (edge false: 1..3) or the core bands (edge: 1..4)
+template <int NJB>
+const void *soloOf(int nvt, bool edge, bool queue)
+{
+    if (edge) switch (nvt) {
+        case 1: return soloKernel<1, NJB, 2>(queue);
+        case 2: return soloKernel<2, NJB, 2>(queue);
+        case 3: return soloKernel<3, NJB, 2>(queue);
+        case 4: return soloKernel<4, NJB, 2>(queue);
+        default: return nullptr;
+        }
     switch (nvt) {
-    case 1: return queue ? reinterpret_cast<const void *>(int8SoloQueue<1, NJB>) : reinterpret_cast<const void *>(int8Solo<1, NJB>);
-    case 2: return queue ? reinterpret_cast<const void *>(int8SoloQueue<2, NJB>) : reinterpret_cast<const void *>(int8Solo<2, NJB>);
-    case 3: return queue ? reinterpret_cast<const void *>(int8SoloQueue<3, NJB>) : reinterpret_cast<const void *>(int8Solo<3, NJB>);
+    case 1: return soloKernel<1, NJB, 0>(queue);
+    case 2: return soloKernel<2, NJB, 0>(queue);
+    case 3: return soloKernel<3, NJB, 0>(queue);
     default: return nullptr;
     }
 }
@@ -890,9 +935,11 @@ Int8Layout int8Layout(const Geometry &g)
     l.quadCrows = g.L;
     l.quadShmem = int8QuadShmem(l.ni4, l.quadTstride, l.quadCrows, l.cstride, l.njb, g);
     l.quadOk = l.supported && l.edge && l.quadShmem <= 80 * 1024 && int8QuadKernel(l.nvt - 1, g.W, l.njb, false);
-    // one wave per window: the padded layout with one screen part
-    l.soloShmem = size_t(int8Lds(l.ni4, l.tstride, l.crows, l.cstride, l.njb, g.M, g.stat, 1).total);
-    l.soloOk = l.supported && l.nvt <= 3 && l.njb <= 5;
+    // one wave per window: the four-wave kernel's template rows, the chip's L rows (edge form) or the full
+    // bands' reach, one screen part
+    l.soloCrows = l.edge ? g.L : l.crows;
+    l.soloShmem = size_t(int8Lds(l.ni4, l.quadTstride, l.soloCrows, l.cstride, l.njb, g.M, g.stat, 1).total);
+    l.soloOk = l.supported && l.njb <= 5 && (l.edge ? l.nvt - 1 <= 4 : l.nvt <= 3);
     return l;
 }
 
@@ -920,9 +967,9 @@ Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue)
     if (l.soloOk && variant == Int8Variant::Solo) {
         Int8Kernel k{};
         switch (l.njb) {
-        case 2: k.func = soloOf<2>(l.nvt, queue); break;
-        case 3: k.func = soloOf<3>(l.nvt, queue); break;
-        case 5: k.func = soloOf<5>(l.nvt, queue); break;
+        case 2: k.func = soloOf<2>(l.edge ? l.nvt - 1 : l.nvt, l.edge, queue); break;
+        case 3: k.func = soloOf<3>(l.edge ? l.nvt - 1 : l.nvt, l.edge, queue); break;
+        case 5: k.func = soloOf<5>(l.edge ? l.nvt - 1 : l.nvt, l.edge, queue); break;
         default: break;
         }
         k.threads = kWave;
@@ -953,12 +1000,12 @@ Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue)
 void fillInt8Args(Int8Args &a, const Int8Layout &l, Int8Variant variant)
 {
     const bool wide = variant == Int8Variant::Wide && l.wideOk, quad = variant == Int8Variant::Quad && l.quadOk;
-    const bool edge = variant == Int8Variant::Simple && l.edge;
+    const bool edge = variant == Int8Variant::Simple && l.edge, solo = variant == Int8Variant::Solo && l.soloOk;
     a.nut = wide ? l.nb : l.nut;
     a.ni4 = l.ni4;
-    a.tstride = wide ? l.wideTstride : quad ? l.quadTstride : edge ? l.edgeTstride : l.tstride;
+    a.tstride = wide ? l.wideTstride : quad || solo ? l.quadTstride : edge ? l.edgeTstride : l.tstride;
     a.cstride = wide ? l.wideCstride : l.cstride;
-    a.crows = wide ? l.wideCrows : quad ? l.quadCrows : edge ? l.edgeCrows : l.crows;
+    a.crows = wide ? l.wideCrows : quad ? l.quadCrows : solo ? l.soloCrows : edge ? l.edgeCrows : l.crows;
     a.alias = l.alias ? 1 : 0;
 }
 

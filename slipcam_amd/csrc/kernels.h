@@ -84,8 +84,10 @@ struct Int8Layout {
     bool quadOk;
     int quadTstride, quadCrows;
     size_t quadShmem;
-    // one wave per window over the full bands (as Padded) where M <= 48 and W <= 64: the LDS of its block
+    // one wave per window where W <= 64 and M = 16 c + 1 <= 65 (c bands, the edge row and column) or
+    // M <= 48 (full bands): its chip rows and the LDS of its block
     bool soloOk;
+    int soloCrows;
     size_t soloShmem;
 };
 
@@ -94,8 +96,8 @@ struct Int8Layout {
 // in LDS; Padded: the one-frame kernel with ceil(M / 16) full bands (no edge waves); Wide: the
 // one-frame kernel on 32 x 32 tiles (v_mfma_i32_32x32x32_i8) where M = 32 c + 1, c <= 4 (else as Auto);
 // Quad: blocks of four waves with equal MFMA shares, software-pipelined over frames, where M = 16 c + 1,
-// c = 4 or 6, and W = 32 or 64 (else as Auto); Solo: one wave per window and frame, full bands, where
-// M <= 48 and W <= 64 (else as Auto)
+// c = 4 or 6, and W = 32 or 64 (else as Auto); Solo: one wave per window and frame where W <= 64 and
+// M = 16 c + 1 <= 65 (c bands, the edge row and column) or M <= 48 (full bands) (else as Simple)
 enum class Int8Variant { Auto = 0, Simple = 1, Piped = 2, Padded = 3, Wide = 4, Quad = 5, Solo = 6 };
 
 Int8Layout int8Layout(const Geometry &g);

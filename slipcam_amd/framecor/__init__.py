@@ -203,9 +203,9 @@ class Correlator(_framecor.Correlator):
         16-row bands even where only an edge row and column remain), 'wide' (one frame at a time, 32 x 32
         tiles, search ranges 16, 32, 48, 64; else as 'simple'), 'quad' (one frame at a time in blocks of
         four waves with equal MFMA shares, search ranges 32 and 48 with windows of 32 or 64; else as
-        'simple') or 'solo' (one wave per window and frame, search ranges up to 23 with windows up to 64;
-        else as 'simple'); every variant gives the same offsets, peaks and covariances, the SNR to
-        rounding"""
+        'simple') or 'solo' (one wave per window and frame, windows up to 64 with search ranges up to 23
+        or of 24 and 32; else as 'simple'); every variant gives the same offsets, peaks and covariances,
+        the SNR to rounding"""
         _check(_lib.framecor_set_variant(self._c, _VARIANTS[variant]))
 
     def set_timing(self, enable=True):
