@@ -488,9 +488,11 @@ struct framecor_batch {
                 float best = 1e30f;
                 Shape keep = sh;
                 int lastTrial = -1;
+                // resident blocks per CU: one-wave blocks 8-16, blocks of four or more waves 1-4
+                static const std::vector<int> fewBlocks{1, 2, 3, 4}, manyBlocks{8, 12, 16};
                 for (int v : variants)
                     for (int fpb : {1, 2, 4, 8, 16, 32})
-                        for (int perCu : {1, 2, 3, 4, 6, 8, 12, 16}) {
+                        for (int perCu : v == int(Int8Variant::Solo) ? manyBlocks : fewBlocks) {
                             Shape cand = sh;
                             cand.variant = v;
                             if (cand.variant != lastTrial) {
