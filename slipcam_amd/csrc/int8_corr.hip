@@ -584,10 +584,10 @@ __device__ __attribute__((always_inline)) inline void int8SoloWindow(const Int8A
     }
 }
 
-// The register budget: three full bands, or the edge form's 2 x 2 or 3 x 3 core tiles, three waves per
-// SIMD (168 registers; at 128 they spill); the edge form's 4 x 4 core tiles two (256 registers), the
-// LDS of its blocks (16 KB at window 32) leaving room for ten per CU anyway
-constexpr int soloWavesPerEu(int nvt, int ew) { return ew != 0 && nvt >= 4 ? 2 : ew != 0 && nvt >= 2 ? 3 : nvt >= 3 ? 3 : 4; }
+// The register budget: up to two full bands or the edge form's 2 x 2 core tiles four waves per SIMD (128
+// registers; 10 KB of LDS at window 32 leaves room for sixteen blocks per CU); three full bands or 3 x 3
+// core tiles three (168); 4 x 4 core tiles two (about 200 registers; at 168 they spill)
+constexpr int soloWavesPerEu(int nvt, int ew) { return ew != 0 && nvt >= 4 ? 2 : nvt >= 3 ? 3 : 4; }
 
 template <int NVT, int NJB, int EW>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(soloWavesPerEu(NVT, EW)))) int8Solo(Int8Args a)

@@ -126,6 +126,10 @@ __device__ inline bool blockPeak(WaveBest *parts, int nw, int wave, int lane, co
     double dbest = -3.0;
     int dib = 0x7fffffff;
     visit([&](int raw, int u, int v) {
+        // each point's FP64 numerator recomputed here from a raw value the compiler cannot match to the
+        // screen's: else it keeps every point's numerator live from the screen to this rarely taken path
+        // (two registers a point: the four-wave kernel's c = 4 form spilled 72 bytes a lane, none now)
+        asm volatile("" : "+v"(raw));
         const int o = u * bw + v;
         const int2 p = scr[o];
         if (screenNcc(raw, p.x, __int_as_float(p.y), mom) >= thr)
