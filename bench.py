@@ -42,8 +42,8 @@ FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector (packed), TFLOP/s
 # that turn them into bytes: what the same counters read for a plain 256 MiB device copy
 # (profiles/counters_int8Quad_fastCorrelate_720p_64frames.md holds the passes, the commands and the commit
 # they were measured on; measure again whenever a kernel changes)
-TRAFFIC_KIB = {'int8': (3.623e6, 1.708e6), 'fft': (1.676e6, 84680.0)}
-TRAFFIC_SOURCE = 'profiles/counters_int8Quad_fastCorrelate_720p_64frames.md (commit dec4a74)'
+TRAFFIC_KIB = {'int8': (3.033e6, 1.201e6), 'fft': (1.676e6, 84680.0)}
+TRAFFIC_SOURCE = 'profiles/counters_int8Quad_fastCorrelate_720p_64frames.md (commit 94ed00f)'
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 TRAFFIC_FRAMES = 64
 
