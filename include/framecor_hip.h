@@ -129,10 +129,10 @@ int framecor_ncc_backward(framecor_correlator *c, const float *frames, int n, co
 int framecor_ncc_refine(framecor_correlator *c, const float *frames, int n, const int32_t *gross,
                         const float *shifts_in, int steps, float *shifts_out, float *ncc, void *stream);
 
-/* the kernel of the int8 method (no counterpart in the reference; a tuning knob): 0 automatic (as 5
- * where it applies, else as 1), 1 one frame at a time on 16 x 16 tiles, 2 the pipelined
- * producer/helper kernel (falls back to 1 where its LDS buffers do not fit), 3 one frame at a time with
- * ceil(M / 16) full 16-row bands (where M = 2 search + 1 = 16 c + 1, variant 1 computes c x c tiles plus
+/* the kernel of the int8 method (no counterpart in the reference; a tuning knob): 0 automatic (as 6
+ * where it applies and the window is at most 33, else as 5 where it applies, else as 1), 1 one frame
+ * at a time on 16 x 16 tiles, 2 the pipelined producer/helper kernel (falls back to 1 where its LDS
+ * buffers do not fit), 3 one frame at a time with ceil(M / 16) full 16-row bands (where M = 2 search + 1 = 16 c + 1, variant 1 computes c x c tiles plus
  * the edge row and the edge column in a wave each), 4 one frame at a time on 32 x 32 tiles
  * (v_mfma_i32_32x32x32_i8) where M = 32 c + 1, c <= 4 (search 16, 32, 48, 64; else as 1), 5 one frame
  * at a time in blocks of four waves with equal MFMA shares where M = 16 c + 1, c = 4 or 6 (search 32 or

@@ -91,7 +91,7 @@ struct Int8Layout {
     size_t soloShmem;
 };
 
-// Auto: Quad where it applies, else Simple; Simple: the one-frame-at-a-time kernel (two blocks per CU),
+// Auto: Solo where it applies and W <= 33, else Quad where it applies, else Simple; Simple: the one-frame-at-a-time kernel (two blocks per CU),
 // with the edge row and the edge column in a wave each where M = 16 c + 1; Piped: the producer/helper pipeline where its buffers fit
 // in LDS; Padded: the one-frame kernel with ceil(M / 16) full bands (no edge waves); Wide: the
 // one-frame kernel on 32 x 32 tiles (v_mfma_i32_32x32x32_i8) where M = 32 c + 1, c <= 4 (else as Auto);

@@ -198,7 +198,8 @@ class Correlator(_framecor.Correlator):
         super().set_reference(frame)
 
     def set_variant(self, variant):
-        """The int8 kernel: 'auto' (as 'quad' where it applies, else 'simple'), 'simple' (one frame at a
+        """The int8 kernel: 'auto' (as 'solo' where it applies with windows up to 33, else as 'quad' where
+        it applies, else 'simple'), 'simple' (one frame at a
         time, 16 x 16 tiles), 'piped' (producer/helper pipeline), 'padded' (one frame at a time, full
         16-row bands even where only an edge row and column remain), 'wide' (one frame at a time, 32 x 32
         tiles, search ranges 16, 32, 48, 64; else as 'simple'), 'quad' (one frame at a time in blocks of
