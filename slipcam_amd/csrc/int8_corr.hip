@@ -976,6 +976,7 @@ Int8Kernel int8Kernel(const Int8Layout &l, Int8Variant variant, bool queue)
         case 5: k.func = soloOf<5>(l.edge ? l.nvt - 1 : l.nvt, l.edge, queue); break;
         default: break;
         }
+        if (!k.func) throw std::invalid_argument("framecor: no one-wave int8 kernel for this window and search range");
         k.threads = kWave;
         k.shmem = l.soloShmem;
         k.variant = Int8Variant::Solo;
