@@ -134,11 +134,12 @@ def test_int8_batch_plans_search_32_and_48(gpu, variant):
 
 
 def test_int8_batch_one_wave_plans(gpu):
-    """The one-wave kernel's shapes (search up to 23, windows up to 64: one, two and three 16-row bands)
+    """The one-wave kernel's shapes (windows up to 64; one, two and three full 16-row bands at search 4,
+    12 and 23; two, three and four core bands with the edge row and column at search 16, 24 and 32)
     under explicit plans, 1 and 16 frames per item, 8 and 12 blocks per CU: the same bits as each clip
     alone with the same kernel and as the edge-wave kernel"""
     specs = []
-    for i, (W, R) in enumerate(((16, 4), (32, 12), (32, 16), (64, 16), (48, 23))):
+    for i, (W, R) in enumerate(((16, 4), (32, 12), (32, 16), (64, 16), (48, 23), (48, 24), (32, 32))):
         specs.append(synthetic.clip_specs(n=1, seed=160 + i, frames=(5, 5), resolutions=((480, 640),), windows=(W,),
                                           searches=(R,))[0])
     vids = videos(specs)
